@@ -1,0 +1,167 @@
+/*
+ * unitair_amd.h -- C ABI of libunitair_amd.so, the MI355X (gfx950) gate
+ * application library behind the `unitair_amd.simulation` Python surface.
+ *
+ * Plain pointers and sizes only.  Every pointer argument that names state,
+ * operator, angle or result data is a device pointer (HBM) on the current
+ * HIP device; `stream` is a hipStream_t (NULL = the legacy default stream).
+ * No entry point synchronises, allocates device memory, or keeps a pointer
+ * after it returns; every launch goes onto `stream` and can be captured into
+ * a HIP graph.  Return value: UA_OK (0) or an error code, with a message in
+ * ua_last_error() (thread-local).
+ *
+ * Data layout (unitair "vector layout", unitair/states/shapes.py:13-27,
+ * unitair/states/conversions.py:5-92): a batch of states is `batch`
+ * contiguous vectors of 2**num_qubits elements.  Complex elements are
+ * interleaved (re, im) pairs, as torch.complex64/complex128 store them.
+ * Qubit q is index bit num_qubits-1-q (qubit 0 = most significant bit).
+ *
+ * Operator layout (unitair/simulation/operations.py:45-148): a (2**k, 2**k)
+ * row-major matrix acting as out = U @ in; for qubits=(q_1, ..., q_k) the
+ * first listed qubit is the most significant bit of the gate index.
+ */
+#ifndef UNITAIR_AMD_H
+#define UNITAIR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UA_ABI_VERSION 1
+
+/* Element types (torch.float32, float64, complex64, complex128). */
+enum {
+  UA_FLOAT32 = 0,
+  UA_FLOAT64 = 1,
+  UA_COMPLEX64 = 2,
+  UA_COMPLEX128 = 3
+};
+
+/* Status codes. */
+enum {
+  UA_OK = 0,
+  UA_ERR_INVALID_ARGUMENT = 1, /* shapes, qubits, dtypes or sizes rejected */
+  UA_ERR_UNSUPPORTED = 2,      /* valid request this build does not handle */
+  UA_ERR_HIP = 3               /* a HIP runtime call or launch failed */
+};
+
+typedef void* ua_stream_t; /* hipStream_t */
+
+/* ABI version of the loaded library (UA_ABI_VERSION at build time). */
+int ua_abi_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* ua_last_error(void);
+
+/*
+ * ua_apply_operator -- replaces unitair.simulation.apply_operator
+ * (unitair/simulation/operations.py:45-148), i.e. its tensor-layout core
+ * apply_operator_tensor (operations.py:151-186) and act_first_qubits_tensor
+ * (operations.py:258-329); act_first_qubits (operations.py:236-255) is the
+ * call with qubits = {0, ..., k-1}.
+ *
+ *   out[b] = (op[b] acting on `qubits`) state[b]      for b in [0, batch)
+ *
+ * out       : batch * 2**num_qubits elements, contiguous.  May equal `state`
+ *             (in place) when state_batch_stride is 2**num_qubits or batch 1.
+ * state     : input states; state b starts at state + b*state_batch_stride
+ *             (stride 0: one state broadcast against a batch of operators,
+ *             the expand() case of operations.py:304-309).
+ * op        : operator b starts at op + b*op_batch_stride; stride 0 = one
+ *             operator for the whole batch.  Same dtype as the state
+ *             (operations.py:322 einsum requires it).
+ * qubits    : op_num_qubits distinct qubit indices in [0, num_qubits), in the
+ *             order of the gate index bits (first = most significant).
+ * Strides are in elements.
+ */
+int ua_apply_operator(void* out, const void* state, const void* op, int dtype,
+                      int num_qubits, int op_num_qubits, const int* qubits,
+                      int64_t batch, int64_t state_batch_stride,
+                      int64_t op_batch_stride, ua_stream_t stream);
+
+/*
+ * ua_apply_phase -- replaces unitair.simulation.apply_phase
+ * (unitair/simulation/operations.py:15-42):
+ *
+ *   out = exp(-1j * angles) * state          (torch broadcasting)
+ *
+ * The output has `rank` dims of sizes `shape` and is written contiguously.
+ * state_strides / angle_strides give, per output dim, the element stride of
+ * each input (0 where that input is broadcast).  The phase factor is formed
+ * in the precision of the angles (complex64 for float32 angles, as torch's
+ * exp(-1j*angles) is), then both factors are promoted to out_dtype and
+ * multiplied.  out_dtype must be complex; rank <= 8.
+ */
+int ua_apply_phase(void* out, int out_dtype, const void* state, int state_dtype,
+                   const void* angles, int angle_dtype, int rank,
+                   const int64_t* shape, const int64_t* state_strides,
+                   const int64_t* angle_strides, ua_stream_t stream);
+
+/*
+ * ua_apply_all_qubits -- replaces unitair.simulation.apply_all_qubits
+ * (unitair/simulation/operations.py:332-366) and apply_all_qubits_tensor
+ * (operations.py:369-413): the same 2x2 operator on every qubit,
+ * out[b] = (op[b] (x) ... (x) op[b]) state[b].  Strides as ua_apply_operator.
+ */
+int ua_apply_all_qubits(void* out, const void* state, const void* op, int dtype,
+                        int num_qubits, int64_t batch,
+                        int64_t state_batch_stride, int64_t op_batch_stride,
+                        ua_stream_t stream);
+
+/*
+ * Reductions over the last dimension with float64 accumulation; replace
+ * unitair/states/innerprod.py.  `length` elements per row, `batch` rows.
+ * Each input row r starts at ptr + r*row_stride (0 = broadcast).  The result
+ * (batch values, contiguous) is written in out_dtype.  `workspace` must hold
+ * ua_reduce_workspace_bytes(batch, length) bytes of device memory.
+ */
+int64_t ua_reduce_workspace_bytes(int64_t batch, int64_t length);
+
+/* norm_squared (innerprod.py:29-46): sum_x |state_x|^2.  out_dtype real. */
+int ua_norm_squared(void* out, int out_dtype, const void* state, int dtype,
+                    int64_t batch, int64_t length, int64_t row_stride,
+                    void* workspace, ua_stream_t stream);
+
+/* inner_product (innerprod.py:62-65): sum_x conj(a_x) b_x.  out_dtype
+ * complex; a and b share `dtype`. */
+int ua_inner_product(void* out, int out_dtype, const void* a, int64_t a_row_stride,
+                     const void* b, int64_t b_row_stride, int dtype,
+                     int64_t batch, int64_t length, void* workspace,
+                     ua_stream_t stream);
+
+/* diag_expectation_value (innerprod.py:49-59): sum_x |state_x|^2 diag_x.
+ * diag is real (float32/float64) or complex; out_dtype follows torch's
+ * promotion of abs_squared(state) * diag. */
+int ua_diag_expectation(void* out, int out_dtype, const void* diag, int diag_dtype,
+                        int64_t diag_row_stride, const void* state, int dtype,
+                        int64_t state_row_stride, int64_t batch, int64_t length,
+                        void* workspace, ua_stream_t stream);
+
+/* abs_squared (innerprod.py:4-26): out_x = |state_x|^2, real, numel values. */
+int ua_abs_squared(void* out, const void* state, int dtype, int64_t numel,
+                   ua_stream_t stream);
+
+/*
+ * In-place circuit gates.  `batch` contiguous states of 2**num_qubits
+ * elements are updated in place; only the amplitudes the gate changes are
+ * read and written.  Results are bit-identical to ua_apply_operator with the
+ * reference's matrices.
+ *
+ * ua_apply_cz   -- apply_operator(cz(), (qubit_a, qubit_b), state)
+ *                  (unitair/gates/gates.py:195-213); also one pair of
+ *                  unitair.simulation.multi_cz (operations.py:657-748).
+ * ua_apply_cnot -- apply_operator(cnot(), (control, target), state)
+ *                  (unitair/gates/gates.py:170-188, control first).
+ */
+int ua_apply_cz(void* state, int dtype, int num_qubits, int qubit_a, int qubit_b,
+                int64_t batch, ua_stream_t stream);
+int ua_apply_cnot(void* state, int dtype, int num_qubits, int control, int target,
+                  int64_t batch, ua_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UNITAIR_AMD_H */

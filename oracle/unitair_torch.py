@@ -1,0 +1,72 @@
+"""Torch CPU restatement of unitair's gate-application op sequence.
+
+TEST INFRASTRUCTURE ONLY: the timed CPU baseline in bench.py ("port" of the
+reference's torch path, run on the host cores) and, being differentiable,
+the source of autograd gradients that the adjoint tests compare with.
+It replays the reference's torch calls step for step:
+  apply_operator   operations.py:45-186  view -> permute().contiguous() ->
+                   einsum('ab...,b...->a...') (operations.py:322) ->
+                   inverse permute().contiguous() -> view
+  apply_phase      operations.py:41-42   exp(-1j*angles) * state
+  norm / <Z>       states/innerprod.py:29-59
+"""
+from __future__ import annotations
+
+from typing import Sequence
+
+import torch
+
+
+def _to_front(n: int, qubits: Sequence[int]):
+    perm = list(qubits) + [q for q in range(n) if q not in qubits]   # utils.py:5-27
+    inv = [0] * n
+    for i, p in enumerate(perm):                                      # utils.py:30-49
+        inv[p] = i
+    return perm, inv
+
+
+def act_first_qubits_tensor(op: torch.Tensor, st: torch.Tensor, n: int, k: int):
+    """operations.py:258-329."""
+    D = 2 ** k
+    sbd, obd = st.dim() - n, op.dim() - 2
+    sb = st.shape[:sbd]
+    if obd > 0 and sbd == 0:
+        st = st.expand(*op.shape[:obd], *st.shape)
+        sbd, sb = obd, op.shape[:obd]
+    st = st.permute(list(range(sbd, st.dim())) + list(range(sbd)))
+    o = op.permute(list(range(obd, op.dim())) + list(range(obd)))
+    old = st.shape
+    res = torch.einsum("ab...,b...->a...", o, st.reshape((D,) + (2,) * (n - k) + tuple(sb)))
+    res = res.reshape(old)
+    d = res.dim()
+    return res.permute(list(range(d - sbd, d)) + list(range(d - sbd)))
+
+
+def apply_operator(op: torch.Tensor, qubits: Sequence[int], state: torch.Tensor):
+    """operations.py:45-186."""
+    N = state.shape[-1]
+    n = N.bit_length() - 1
+    k = len(qubits)
+    perm, inv = _to_front(n, list(qubits))
+    sbd = state.dim() - 1
+    st = state.reshape(state.shape[:-1] + (2,) * n)
+    st = st.permute(list(range(sbd)) + [sbd + p for p in perm]).contiguous()
+    st = act_first_qubits_tensor(op, st, n, k)
+    sbd = st.dim() - n
+    st = st.permute(list(range(sbd)) + [sbd + p for p in inv]).contiguous()
+    return st.reshape(st.shape[:sbd] + (N,))
+
+
+def apply_phase(angles: torch.Tensor, state: torch.Tensor):
+    """operations.py:15-42."""
+    return torch.exp(-1.0j * angles) * state
+
+
+def norm_squared(state: torch.Tensor):
+    """innerprod.py:29-46."""
+    return torch.sum((state.conj() * state).real, dim=-1)
+
+
+def diag_expectation_value(diag: torch.Tensor, state: torch.Tensor):
+    """innerprod.py:49-59."""
+    return torch.sum((state.conj() * state).real * diag, dim=-1)

@@ -1,0 +1,365 @@
+// Dense k-qubit gate application: ua_apply_operator.
+//
+// Replaces the reference's permute -> einsum -> inverse-permute sequence
+// (unitair/simulation/operations.py:151-186 and :258-329), which moves the
+// whole state three to five times per gate, with one streaming pass that
+// addresses the target bits in place:
+//
+//   out[i] = sum_j U[r(i), j] * in[(i & ~targets) | scatter(j)]
+//
+// where r(i) gathers the target bits of i into a gate index whose most
+// significant bit belongs to the first listed qubit (operations.py:78-86,
+// unitair/utils.py:5-27).
+//
+// Fast path (k <= 5): one thread owns whole groups of 2**k amplitudes (the
+// amplitudes that differ only in target bits), so a group is read once and
+// written once and the kernel can run in place.  On the host the targets are
+// sorted by state bit ("canonical" order) and the matrix is read through the
+// matching index permutation, so the lowest target is always gate bit 0.
+// That fixes the access pattern at compile time:
+//   VEC  : lowest target bit >= 1 -> each thread takes two neighbouring groups
+//          and moves 2 amplitudes per access (16 B for complex64);
+//   PAIR : lowest target bit == 0 -> the two amplitudes that differ in bit 0
+//          are adjacent and move as one 16 B access;
+//   SCALAR: fallback for misaligned views.
+// For k >= 3 the (permuted) matrix is staged in LDS once per workgroup.
+#include "ua_common.h"
+
+namespace ua {
+namespace {
+
+constexpr int kMaxFastK = 5;
+
+enum Mode { SCALAR = 0, VEC = 1, PAIR = 2 };
+
+struct FastParams {
+  int64_t total;        // threads that own work
+  int64_t in_bstride;   // elements between input states
+  int64_t out_bstride;  // elements between output states
+  int64_t op_bstride;   // elements between operators
+  int log2_tps;         // log2(threads per state)
+  int tb[kMaxFastK];    // ascending target bit positions
+  int gpos[kMaxFastK];  // gate-index bit of canonical bit p
+};
+
+// Original (caller's) gate index of canonical index c.
+template <int K>
+__device__ __forceinline__ int orig_index(int c, const FastParams& p) {
+  int r = 0;
+#pragma unroll
+  for (int q = 0; q < K; ++q) r |= ((c >> q) & 1) << p.gpos[q];
+  return r;
+}
+
+// State offset of canonical gate index j (j is a compile-time constant at
+// every call site, so this folds to a few adds).
+template <int K>
+__device__ __forceinline__ uint64_t offset_of(int j, const FastParams& p) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    if ((j >> q) & 1) o += uint64_t(1) << p.tb[q];
+  return o;
+}
+
+template <typename T, int K, int MODE, bool LDS>
+__global__ __launch_bounds__(kBlock) void dense_fast_kernel(
+    T* out, const T* in, const T* __restrict__ op, FastParams p) {
+  using A = Num<T>;
+  constexpr int D = 1 << K;
+  constexpr int G = (MODE == VEC) ? 2 : 1;  // groups per thread
+
+  __shared__ T smat[LDS ? D * D : 1];
+  if constexpr (LDS) {
+    // The host only picks LDS staging when all threads of a workgroup use
+    // the same operator.
+    const int64_t b0 = (int64_t(blockIdx.x) * kBlock) >> p.log2_tps;
+    const T* opb = op + b0 * p.op_bstride;
+    for (int e = threadIdx.x; e < D * D; e += kBlock) {
+      const int r = e >> K, c = e & (D - 1);
+      smat[e] = opb[orig_index<K>(r, p) * D + orig_index<K>(c, p)];
+    }
+    __syncthreads();
+  }
+
+  const int64_t tid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (tid >= p.total) return;
+  const int64_t b = tid >> p.log2_tps;
+  const uint64_t t = uint64_t(tid) & ((uint64_t(1) << p.log2_tps) - 1);
+  const uint64_t base = insert_zero_bits(t * G, p.tb, K);
+  const T* src = in + b * p.in_bstride + base;
+  T* dst = out + b * p.out_bstride + base;
+  const T* opb = op + b * p.op_bstride;
+
+  // ---- load the group(s)
+  T a[D][G];
+  if constexpr (MODE == VEC) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const Pair<T> v = *reinterpret_cast<const Pair<T>*>(src + offset_of<K>(j, p));
+      a[j][0] = v.v[0];
+      a[j][1] = v.v[1];
+    }
+  } else if constexpr (MODE == PAIR) {
+#pragma unroll
+    for (int j = 0; j < D; j += 2) {
+      const Pair<T> v = *reinterpret_cast<const Pair<T>*>(src + offset_of<K>(j, p));
+      a[j][0] = v.v[0];
+      a[j + 1][0] = v.v[1];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < D; ++j) a[j][0] = src[offset_of<K>(j, p)];
+  }
+
+  // ---- matrix element accessor (canonical order)
+  T ureg[(LDS || K > 2) ? 1 : D * D];
+  if constexpr (!LDS && K <= 2) {
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        ureg[r * D + c] = opb[orig_index<K>(r, p) * D + orig_index<K>(c, p)];
+  }
+  auto U = [&](int r, int c) -> T {
+    if constexpr (LDS) {
+      return smat[r * D + c];
+    } else if constexpr (K <= 2) {
+      return ureg[r * D + c];
+    } else {
+      return opb[orig_index<K>(r, p) * D + orig_index<K>(c, p)];
+    }
+  };
+
+  // ---- multiply and store
+  if constexpr (MODE == PAIR) {
+#pragma unroll
+    for (int r = 0; r < D; r += 2) {
+      T acc0 = A::zero(), acc1 = A::zero();
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        acc0 = A::mac(acc0, U(r, j), a[j][0]);
+        acc1 = A::mac(acc1, U(r + 1, j), a[j][0]);
+      }
+      Pair<T> v;
+      v.v[0] = acc0;
+      v.v[1] = acc1;
+      *reinterpret_cast<Pair<T>*>(dst + offset_of<K>(r, p)) = v;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      T acc[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) acc[g] = A::zero();
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const T u = U(r, j);
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g] = A::mac(acc[g], u, a[j][g]);
+      }
+      if constexpr (MODE == VEC) {
+        Pair<T> v;
+        v.v[0] = acc[0];
+        v.v[1] = acc[1];
+        *reinterpret_cast<Pair<T>*>(dst + offset_of<K>(r, p)) = v;
+      } else {
+        dst[offset_of<K>(r, p)] = acc[0];
+      }
+    }
+  }
+}
+
+// Any k (including 0 and k > 5): one thread per output amplitude, reading
+// its group from memory.  Out of place only.
+constexpr int kMaxQubits = 62;
+
+struct GenericParams {
+  int64_t total;  // batch * 2**n
+  int64_t in_bstride, op_bstride;
+  int n, k;
+  uint64_t tmask;        // OR of all target bits
+  int tbit[kMaxQubits];  // state bit of gate bit k-1-m, in `qubits` order
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dense_generic_kernel(
+    T* __restrict__ out, const T* __restrict__ in, const T* __restrict__ op,
+    GenericParams p) {
+  using A = Num<T>;
+  const int64_t tid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (tid >= p.total) return;
+  const int64_t b = tid >> p.n;
+  const uint64_t x = uint64_t(tid) & ((uint64_t(1) << p.n) - 1);
+  const int64_t D = int64_t(1) << p.k;
+  int64_t r = 0;
+  for (int m = 0; m < p.k; ++m) r = (r << 1) | int64_t((x >> p.tbit[m]) & 1);
+  const uint64_t base = x & ~p.tmask;
+  const T* src = in + b * p.in_bstride;
+  const T* row = op + b * p.op_bstride + r * D;
+  T acc = A::zero();
+  for (int64_t j = 0; j < D; ++j) {
+    uint64_t s = base;
+    for (int m = 0; m < p.k; ++m)
+      if ((j >> (p.k - 1 - m)) & 1) s |= uint64_t(1) << p.tbit[m];
+    acc = A::mac(acc, row[j], src[s]);
+  }
+  out[tid] = acc;
+}
+
+template <typename T, int K, int MODE, bool LDS>
+void launch_fast(void* out, const void* in, const void* op, const FastParams& p,
+                 hipStream_t s) {
+  dense_fast_kernel<T, K, MODE, LDS><<<grid_for(p.total), kBlock, 0, s>>>(
+      static_cast<T*>(out), static_cast<const T*>(in), static_cast<const T*>(op), p);
+}
+
+template <typename T, int K, int MODE>
+void launch_fast_lds(bool lds, void* out, const void* in, const void* op,
+                     const FastParams& p, hipStream_t s) {
+  if constexpr (K >= 3) {
+    if (lds) return launch_fast<T, K, MODE, true>(out, in, op, p, s);
+  }
+  launch_fast<T, K, MODE, false>(out, in, op, p, s);
+}
+
+template <typename T, int K>
+void launch_fast_mode(int mode, bool lds, void* out, const void* in,
+                      const void* op, const FastParams& p, hipStream_t s) {
+  switch (mode) {
+    case VEC: return launch_fast_lds<T, K, VEC>(lds, out, in, op, p, s);
+    case PAIR: return launch_fast_lds<T, K, PAIR>(lds, out, in, op, p, s);
+    default: return launch_fast_lds<T, K, SCALAR>(lds, out, in, op, p, s);
+  }
+}
+
+template <typename T>
+void launch_fast_k(int k, int mode, bool lds, void* out, const void* in,
+                   const void* op, const FastParams& p, hipStream_t s) {
+  switch (k) {
+    case 1: return launch_fast_mode<T, 1>(mode, lds, out, in, op, p, s);
+    case 2: return launch_fast_mode<T, 2>(mode, lds, out, in, op, p, s);
+    case 3: return launch_fast_mode<T, 3>(mode, lds, out, in, op, p, s);
+    case 4: return launch_fast_mode<T, 4>(mode, lds, out, in, op, p, s);
+    case 5: return launch_fast_mode<T, 5>(mode, lds, out, in, op, p, s);
+  }
+}
+
+bool aligned(const void* ptr, int64_t stride_elems, size_t esize, size_t align) {
+  return (reinterpret_cast<uintptr_t>(ptr) % align) == 0 &&
+         ((stride_elems * int64_t(esize)) % int64_t(align)) == 0;
+}
+
+}  // namespace
+
+// Validated entry used by ua_apply_operator and by other host code.
+int apply_operator_impl(void* out, const void* state, const void* op, int dtype,
+                        int n, int k, const int* qubits, int64_t batch,
+                        int64_t state_bstride, int64_t op_bstride, hipStream_t s) {
+  const size_t es = elem_size(dtype);
+  UA_CHECK_ARG(es != 0, "unsupported dtype code %d", dtype);
+  UA_CHECK_ARG(n >= 0 && n <= kMaxQubits, "num_qubits=%d out of range", n);
+  UA_CHECK_ARG(k >= 0 && k <= n, "cannot apply a %d-qubit operator to %d qubit(s)", k, n);
+  UA_CHECK_ARG(batch >= 0, "batch=%lld is negative", (long long)batch);
+  UA_CHECK_ARG(state_bstride >= 0 && op_bstride >= 0, "negative batch stride");
+  UA_CHECK_ARG(k == 0 || qubits != nullptr, "qubits is NULL");
+  if (batch == 0) return UA_OK;
+  UA_CHECK_ARG(out && state && op, "NULL data pointer");
+
+  uint64_t seen = 0;
+  for (int m = 0; m < k; ++m) {
+    UA_CHECK_ARG(qubits[m] >= 0 && qubits[m] < n,
+                 "qubit %d is not in range for a %d-qubit state", qubits[m], n);
+    const uint64_t bit = uint64_t(1) << qubits[m];
+    UA_CHECK_ARG(!(seen & bit), "qubit %d appears more than once", qubits[m]);
+    seen |= bit;
+  }
+  const int64_t N = int64_t(1) << n;
+  const bool in_place = (out == state);
+  UA_CHECK_ARG(!in_place || state_bstride == N || batch == 1,
+               "in-place application needs one state per batch entry");
+
+  if (k >= 1 && k <= kMaxFastK) {
+    FastParams p{};
+    // canonical order: ascending state bit; gpos = caller's gate-index bit
+    int tb[kMaxFastK], gp[kMaxFastK];
+    for (int m = 0; m < k; ++m) {
+      tb[m] = n - 1 - qubits[m];
+      gp[m] = k - 1 - m;
+    }
+    for (int i = 1; i < k; ++i)  // insertion sort by tb
+      for (int j = i; j > 0 && tb[j - 1] > tb[j]; --j) {
+        int t0 = tb[j]; tb[j] = tb[j - 1]; tb[j - 1] = t0;
+        int g0 = gp[j]; gp[j] = gp[j - 1]; gp[j - 1] = g0;
+      }
+    for (int m = 0; m < k; ++m) { p.tb[m] = tb[m]; p.gpos[m] = gp[m]; }
+
+    const size_t pair_align = 2 * es;
+    const bool io_aligned = aligned(state, state_bstride, es, pair_align) &&
+                            aligned(out, N, es, pair_align);
+    int mode = SCALAR;
+    if (io_aligned) mode = (tb[0] == 0) ? PAIR : VEC;
+    const int log2_groups = n - k;  // groups per state
+    const int log2_tps = log2_groups - (mode == VEC ? 1 : 0);
+    p.log2_tps = log2_tps;
+    p.total = batch << log2_tps;
+    p.in_bstride = state_bstride;
+    p.out_bstride = N;
+    p.op_bstride = op_bstride;
+    const bool lds = (k >= 3) && (op_bstride == 0 || log2_tps >= 8);
+    switch (dtype) {
+      case UA_FLOAT32: launch_fast_k<float>(k, mode, lds, out, state, op, p, s); break;
+      case UA_FLOAT64: launch_fast_k<double>(k, mode, lds, out, state, op, p, s); break;
+      case UA_COMPLEX64: launch_fast_k<float2>(k, mode, lds, out, state, op, p, s); break;
+      case UA_COMPLEX128: launch_fast_k<double2>(k, mode, lds, out, state, op, p, s); break;
+    }
+    return check_launch("dense_fast_kernel");
+  }
+
+  UA_CHECK_ARG(!in_place, "in-place application supports operators on 1..%d qubits",
+               kMaxFastK);
+  UA_CHECK_ARG(k <= 20, "operators on more than 20 qubits are not supported");
+  GenericParams g{};
+  g.total = batch * N;
+  g.in_bstride = state_bstride;
+  g.op_bstride = op_bstride;
+  g.n = n;
+  g.k = k;
+  g.tmask = 0;
+  for (int m = 0; m < k; ++m) {
+    g.tbit[m] = n - 1 - qubits[m];
+    g.tmask |= uint64_t(1) << g.tbit[m];
+  }
+  const unsigned grid = grid_for(g.total);
+  switch (dtype) {
+    case UA_FLOAT32:
+      dense_generic_kernel<float><<<grid, kBlock, 0, s>>>(
+          (float*)out, (const float*)state, (const float*)op, g);
+      break;
+    case UA_FLOAT64:
+      dense_generic_kernel<double><<<grid, kBlock, 0, s>>>(
+          (double*)out, (const double*)state, (const double*)op, g);
+      break;
+    case UA_COMPLEX64:
+      dense_generic_kernel<float2><<<grid, kBlock, 0, s>>>(
+          (float2*)out, (const float2*)state, (const float2*)op, g);
+      break;
+    case UA_COMPLEX128:
+      dense_generic_kernel<double2><<<grid, kBlock, 0, s>>>(
+          (double2*)out, (const double2*)state, (const double2*)op, g);
+      break;
+  }
+  return check_launch("dense_generic_kernel");
+}
+
+}  // namespace ua
+
+extern "C" int ua_apply_operator(void* out, const void* state, const void* op,
+                                 int dtype, int num_qubits, int op_num_qubits,
+                                 const int* qubits, int64_t batch,
+                                 int64_t state_batch_stride,
+                                 int64_t op_batch_stride, ua_stream_t stream) {
+  return ua::apply_operator_impl(out, state, op, dtype, num_qubits, op_num_qubits,
+                                 qubits, batch, state_batch_stride, op_batch_stride,
+                                 static_cast<hipStream_t>(stream));
+}

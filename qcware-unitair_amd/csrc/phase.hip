@@ -1,0 +1,219 @@
+// Diagonal phase: ua_apply_phase, out = exp(-1j * angles) * state with torch
+// broadcasting (unitair/simulation/operations.py:15-42).
+//
+// The reference runs exp() into a full-size complex temporary and then a
+// broadcast multiply (two kernels, three state-sized passes).  Here one pass
+// reads each amplitude and each angle once, forms cos/sin in registers and
+// writes the product.  `out` may equal `state` (in place) when both are
+// contiguous with the output's shape.
+#include "ua_common.h"
+
+namespace ua {
+namespace {
+
+constexpr int kMaxRank = 8;
+
+enum PhaseMode { CONTIG = 0, TILE = 1, GENERAL = 2 };
+
+struct PhaseParams {
+  int64_t total;
+  int64_t tile;  // TILE: angles repeat every `tile` elements
+  int rank;
+  int64_t shape[kMaxRank];
+  int64_t sstride[kMaxRank];
+  int64_t astride[kMaxRank];
+};
+
+// exp(-1j * a) in the precision of the angle type.
+__device__ __forceinline__ float2 phase_of(float a) {
+  float s, c;
+  sincosf(a, &s, &c);
+  return make_float2(c, -s);
+}
+__device__ __forceinline__ double2 phase_of(double a) {
+  double s, c;
+  sincos(a, &s, &c);
+  return make_double2(c, -s);
+}
+// Complex angle a = x + iy: exp(-1j*a) = exp(y) * (cos x - i sin x).
+__device__ __forceinline__ float2 phase_of(float2 a) {
+  float s, c;
+  sincosf(a.x, &s, &c);
+  const float m = expf(a.y);
+  return make_float2(m * c, -(m * s));
+}
+__device__ __forceinline__ double2 phase_of(double2 a) {
+  double s, c;
+  sincos(a.x, &s, &c);
+  const double m = exp(a.y);
+  return make_double2(m * c, -(m * s));
+}
+
+template <typename TO> struct Widen;
+template <> struct Widen<float2> {
+  __device__ static float2 from(float x) { return make_float2(x, 0.f); }
+  __device__ static float2 from(float2 x) { return x; }
+};
+template <> struct Widen<double2> {
+  __device__ static double2 from(float x) { return make_double2(x, 0.0); }
+  __device__ static double2 from(double x) { return make_double2(x, 0.0); }
+  __device__ static double2 from(float2 x) { return make_double2(x.x, x.y); }
+  __device__ static double2 from(double2 x) { return x; }
+};
+
+template <typename TO>
+__device__ __forceinline__ TO cmul(TO a, TO b) {
+  TO r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+template <typename TO, typename TS, typename TA, int MODE>
+__global__ __launch_bounds__(kBlock) void phase_kernel(
+    TO* out, const TS* state, const TA* __restrict__ angles, PhaseParams p) {
+  const int64_t stride = int64_t(gridDim.x) * kBlock;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < p.total; i += stride) {
+    int64_t si, ai;
+    if constexpr (MODE == CONTIG) {
+      si = i;
+      ai = i;
+    } else if constexpr (MODE == TILE) {
+      si = i;
+      ai = i % p.tile;
+    } else {
+      int64_t rem = i;
+      si = 0;
+      ai = 0;
+      for (int d = p.rank - 1; d >= 0; --d) {
+        const int64_t c = rem % p.shape[d];
+        rem /= p.shape[d];
+        si += c * p.sstride[d];
+        ai += c * p.astride[d];
+      }
+    }
+    const TO ph = Widen<TO>::from(phase_of(angles[ai]));
+    const TO x = Widen<TO>::from(state[si]);
+    out[i] = cmul(ph, x);
+  }
+}
+
+template <typename TO, typename TS, typename TA>
+void launch_phase(int mode, void* out, const void* st, const void* an,
+                  const PhaseParams& p, hipStream_t s) {
+  int64_t blocks = (p.total + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  const unsigned g = static_cast<unsigned>(blocks);
+  TO* o = static_cast<TO*>(out);
+  const TS* x = static_cast<const TS*>(st);
+  const TA* a = static_cast<const TA*>(an);
+  switch (mode) {
+    case CONTIG: phase_kernel<TO, TS, TA, CONTIG><<<g, kBlock, 0, s>>>(o, x, a, p); break;
+    case TILE: phase_kernel<TO, TS, TA, TILE><<<g, kBlock, 0, s>>>(o, x, a, p); break;
+    default: phase_kernel<TO, TS, TA, GENERAL><<<g, kBlock, 0, s>>>(o, x, a, p); break;
+  }
+}
+
+template <typename TO, typename TS>
+int dispatch_angle(int adt, int mode, void* out, const void* st, const void* an,
+                   const PhaseParams& p, hipStream_t s) {
+  switch (adt) {
+    case UA_FLOAT32: launch_phase<TO, TS, float>(mode, out, st, an, p, s); return UA_OK;
+    case UA_COMPLEX64: launch_phase<TO, TS, float2>(mode, out, st, an, p, s); return UA_OK;
+  }
+  if constexpr (sizeof(TO) == 16) {
+    switch (adt) {
+      case UA_FLOAT64: launch_phase<TO, TS, double>(mode, out, st, an, p, s); return UA_OK;
+      case UA_COMPLEX128: launch_phase<TO, TS, double2>(mode, out, st, an, p, s); return UA_OK;
+    }
+  }
+  set_error("angle dtype %d cannot produce a result of this precision", adt);
+  return UA_ERR_INVALID_ARGUMENT;
+}
+
+template <typename TO>
+int dispatch_state(int sdt, int adt, int mode, void* out, const void* st,
+                   const void* an, const PhaseParams& p, hipStream_t s) {
+  switch (sdt) {
+    case UA_FLOAT32: return dispatch_angle<TO, float>(adt, mode, out, st, an, p, s);
+    case UA_COMPLEX64: return dispatch_angle<TO, float2>(adt, mode, out, st, an, p, s);
+  }
+  if constexpr (sizeof(TO) == 16) {
+    switch (sdt) {
+      case UA_FLOAT64: return dispatch_angle<TO, double>(adt, mode, out, st, an, p, s);
+      case UA_COMPLEX128: return dispatch_angle<TO, double2>(adt, mode, out, st, an, p, s);
+    }
+  }
+  set_error("state dtype %d cannot produce a result of this precision", sdt);
+  return UA_ERR_INVALID_ARGUMENT;
+}
+
+bool is_row_major(int rank, const int64_t* shape, const int64_t* stride, int first) {
+  int64_t expect = 1;
+  for (int d = rank - 1; d >= first; --d) {
+    if (shape[d] != 1 && stride[d] != expect) return false;
+    expect *= shape[d];
+  }
+  return true;
+}
+
+}  // namespace
+
+int apply_phase_impl(void* out, int odt, const void* state, int sdt, const void* angles,
+                     int adt, int rank, const int64_t* shape, const int64_t* sstr,
+                     const int64_t* astr, hipStream_t s) {
+  UA_CHECK_ARG(odt == UA_COMPLEX64 || odt == UA_COMPLEX128,
+               "phase output dtype must be complex (got %d)", odt);
+  UA_CHECK_ARG(elem_size(sdt) && elem_size(adt), "unsupported dtype");
+  UA_CHECK_ARG(rank >= 0 && rank <= kMaxRank, "rank %d exceeds %d", rank, kMaxRank);
+  PhaseParams p{};
+  p.rank = rank;
+  p.total = 1;
+  for (int d = 0; d < rank; ++d) {
+    UA_CHECK_ARG(shape[d] >= 0, "negative size");
+    p.shape[d] = shape[d];
+    p.sstride[d] = sstr[d];
+    p.astride[d] = astr[d];
+    p.total *= shape[d];
+  }
+  if (p.total == 0) return UA_OK;
+  UA_CHECK_ARG(out && state && angles, "NULL data pointer");
+
+  int mode = GENERAL;
+  if (is_row_major(rank, p.shape, p.sstride, 0)) {
+    if (is_row_major(rank, p.shape, p.astride, 0)) {
+      mode = CONTIG;
+    } else {
+      // angles contiguous over trailing dims [f, rank) and broadcast before
+      for (int f = 0; f <= rank; ++f) {
+        bool lead_zero = true;
+        for (int d = 0; d < f; ++d) lead_zero &= (p.astride[d] == 0 || p.shape[d] == 1);
+        if (lead_zero && is_row_major(rank, p.shape, p.astride, f)) {
+          int64_t t = 1;
+          for (int d = f; d < rank; ++d) t *= p.shape[d];
+          p.tile = t;
+          mode = TILE;
+          break;
+        }
+      }
+    }
+  }
+  int rc;
+  if (odt == UA_COMPLEX64)
+    rc = dispatch_state<float2>(sdt, adt, mode, out, state, angles, p, s);
+  else
+    rc = dispatch_state<double2>(sdt, adt, mode, out, state, angles, p, s);
+  if (rc != UA_OK) return rc;
+  return check_launch("phase_kernel");
+}
+
+}  // namespace ua
+
+extern "C" int ua_apply_phase(void* out, int out_dtype, const void* state, int state_dtype,
+                              const void* angles, int angle_dtype, int rank,
+                              const int64_t* shape, const int64_t* state_strides,
+                              const int64_t* angle_strides, ua_stream_t stream) {
+  return ua::apply_phase_impl(out, out_dtype, state, state_dtype, angles, angle_dtype,
+                              rank, shape, state_strides, angle_strides,
+                              static_cast<hipStream_t>(stream));
+}

@@ -1,0 +1,118 @@
+// In-place CZ and CNOT for circuit execution.
+//
+// In unitair these gates are dense 4x4 matrices (unitair/gates/gates.py:
+// 170-213) applied through apply_operator, or the CZ-only multi_cz
+// (unitair/simulation/operations.py:657-748).  Both are sign flips or swaps:
+// run in place they touch only the amplitudes they change, a quarter (CZ) or
+// a half (CNOT) of the state, and their results are bit-identical to the
+// dense application (a multiply by 0/1/-1 and exact sums).
+#include "ua_common.h"
+
+namespace ua {
+namespace {
+
+struct SpecialParams {
+  int64_t total;      // threads that own work
+  int64_t bstride;    // elements between states
+  int log2_tps;       // log2(threads per state)
+  int lo, hi;         // ascending bit positions of the two qubits
+  uint64_t set_bits;  // bits forced to 1 in every visited index
+  uint64_t flip;      // CNOT: target bit (0 for CZ)
+};
+
+__device__ __forceinline__ float2 negate(float2 v) { return make_float2(-v.x, -v.y); }
+__device__ __forceinline__ double2 negate(double2 v) { return make_double2(-v.x, -v.y); }
+__device__ __forceinline__ float negate(float v) { return -v; }
+__device__ __forceinline__ double negate(double v) { return -v; }
+
+// One thread per G consecutive indices with both (CZ) or the control (CNOT)
+// bit set.  VEC (G = 2) when bit 0 is not one of the two qubits.
+template <typename T, bool CNOT, int G>
+__global__ __launch_bounds__(kBlock) void special_kernel(T* state, SpecialParams p) {
+  const int64_t tid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (tid >= p.total) return;
+  const int64_t b = tid >> p.log2_tps;
+  const uint64_t t = uint64_t(tid) & ((uint64_t(1) << p.log2_tps) - 1);
+  const int pos[2] = {p.lo, p.hi};
+  const uint64_t i = insert_zero_bits(t * G, pos, 2) | p.set_bits;
+  T* s = state + b * p.bstride;
+  if constexpr (G == 2) {
+    Pair<T>* a = reinterpret_cast<Pair<T>*>(s + i);
+    if constexpr (CNOT) {
+      Pair<T>* c = reinterpret_cast<Pair<T>*>(s + (i | p.flip));
+      const Pair<T> x = *a, y = *c;
+      *a = y;
+      *c = x;
+    } else {
+      Pair<T> x = *a;
+      x.v[0] = negate(x.v[0]);
+      x.v[1] = negate(x.v[1]);
+      *a = x;
+    }
+  } else {
+    if constexpr (CNOT) {
+      const T x = s[i], y = s[i | p.flip];
+      s[i] = y;
+      s[i | p.flip] = x;
+    } else {
+      s[i] = negate(s[i]);
+    }
+  }
+}
+
+template <typename T, bool CNOT>
+void launch_special(void* state, SpecialParams p, bool vec, hipStream_t s) {
+  if (vec)
+    special_kernel<T, CNOT, 2><<<grid_for(p.total), kBlock, 0, s>>>(static_cast<T*>(state), p);
+  else
+    special_kernel<T, CNOT, 1><<<grid_for(p.total), kBlock, 0, s>>>(static_cast<T*>(state), p);
+}
+
+int run_special(bool cnot, void* state, int dtype, int n, int qa, int qb, int64_t batch,
+                hipStream_t s) {
+  const size_t es = elem_size(dtype);
+  UA_CHECK_ARG(es != 0, "unsupported dtype code %d", dtype);
+  UA_CHECK_ARG(n >= 2 && n <= 62, "num_qubits=%d out of range", n);
+  UA_CHECK_ARG(qa >= 0 && qa < n && qb >= 0 && qb < n && qa != qb,
+               "qubits (%d, %d) invalid for %d qubits", qa, qb, n);
+  UA_CHECK_ARG(batch >= 0, "negative batch");
+  if (batch == 0) return UA_OK;
+  UA_CHECK_ARG(state != nullptr, "NULL state");
+  const int ba = n - 1 - qa, bb = n - 1 - qb;  // CNOT: qa control, qb target
+  SpecialParams p{};
+  p.lo = ba < bb ? ba : bb;
+  p.hi = ba < bb ? bb : ba;
+  p.bstride = int64_t(1) << n;
+  if (cnot) {
+    p.set_bits = uint64_t(1) << ba;
+    p.flip = uint64_t(1) << bb;
+  } else {
+    p.set_bits = (uint64_t(1) << ba) | (uint64_t(1) << bb);
+    p.flip = 0;
+  }
+  const bool vec = p.lo >= 1 && (reinterpret_cast<uintptr_t>(state) % (2 * es)) == 0;
+  p.log2_tps = n - 2 - (vec ? 1 : 0);
+  p.total = batch << p.log2_tps;
+  switch (dtype) {
+    case UA_FLOAT32: cnot ? launch_special<float, true>(state, p, vec, s) : launch_special<float, false>(state, p, vec, s); break;
+    case UA_FLOAT64: cnot ? launch_special<double, true>(state, p, vec, s) : launch_special<double, false>(state, p, vec, s); break;
+    case UA_COMPLEX64: cnot ? launch_special<float2, true>(state, p, vec, s) : launch_special<float2, false>(state, p, vec, s); break;
+    case UA_COMPLEX128: cnot ? launch_special<double2, true>(state, p, vec, s) : launch_special<double2, false>(state, p, vec, s); break;
+  }
+  return check_launch(cnot ? "cnot_kernel" : "cz_kernel");
+}
+
+}  // namespace
+}  // namespace ua
+
+extern "C" int ua_apply_cz(void* state, int dtype, int num_qubits, int qubit_a, int qubit_b,
+                           int64_t batch, ua_stream_t stream) {
+  return ua::run_special(false, state, dtype, num_qubits, qubit_a, qubit_b, batch,
+                         static_cast<hipStream_t>(stream));
+}
+
+extern "C" int ua_apply_cnot(void* state, int dtype, int num_qubits, int control, int target,
+                             int64_t batch, ua_stream_t stream) {
+  return ua::run_special(true, state, dtype, num_qubits, control, target, batch,
+                         static_cast<hipStream_t>(stream));
+}

@@ -1,0 +1,120 @@
+"""ctypes binding of libunitair_amd.so (declarations: include/unitair_amd.h).
+
+torch is imported before the library is opened so that the HIP runtime
+already loaded by torch (soname libamdhip64.so.7) is the one the library
+binds to: device pointers and streams handed over from torch are then valid
+inside the library.  There is no fallback: if the library is missing or
+fails to load, every GPU entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede the library load, see above)
+
+LIB_ENV = "UNITAIR_AMD_LIB"
+_DEFAULT_PATH = Path(__file__).resolve().parent / "libunitair_amd.so"
+
+UA_OK = 0
+UA_ERR_INVALID_ARGUMENT = 1
+UA_ERR_UNSUPPORTED = 2
+UA_ERR_HIP = 3
+ABI_VERSION = 1
+
+UA_FLOAT32 = 0
+UA_FLOAT64 = 1
+UA_COMPLEX64 = 2
+UA_COMPLEX128 = 3
+
+DTYPE_CODES = {
+    torch.float32: UA_FLOAT32,
+    torch.float64: UA_FLOAT64,
+    torch.complex64: UA_COMPLEX64,
+    torch.complex128: UA_COMPLEX128,
+}
+
+_i32 = ctypes.c_int
+_i64 = ctypes.c_int64
+_p = ctypes.c_void_p
+_pi32 = ctypes.POINTER(ctypes.c_int)
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+
+# name -> (restype, argtypes); mirrors include/unitair_amd.h
+SIGNATURES = {
+    "ua_abi_version": (_i32, []),
+    "ua_last_error": (ctypes.c_char_p, []),
+    "ua_apply_operator": (_i32, [_p, _p, _p, _i32, _i32, _i32, _pi32, _i64, _i64, _i64, _p]),
+    "ua_apply_phase": (_i32, [_p, _i32, _p, _i32, _p, _i32, _i32, _pi64, _pi64, _pi64, _p]),
+    "ua_apply_all_qubits": (_i32, [_p, _p, _p, _i32, _i32, _i64, _i64, _i64, _p]),
+    "ua_reduce_workspace_bytes": (_i64, [_i64, _i64]),
+    "ua_norm_squared": (_i32, [_p, _i32, _p, _i32, _i64, _i64, _i64, _p, _p]),
+    "ua_inner_product": (_i32, [_p, _i32, _p, _i64, _p, _i64, _i32, _i64, _i64, _p, _p]),
+    "ua_diag_expectation": (_i32, [_p, _i32, _p, _i32, _i64, _p, _i32, _i64, _i64, _i64, _p, _p]),
+    "ua_abs_squared": (_i32, [_p, _p, _i32, _i64, _p]),
+    "ua_apply_cz": (_i32, [_p, _i32, _i32, _i32, _i32, _i64, _p]),
+    "ua_apply_cnot": (_i32, [_p, _i32, _i32, _i32, _i32, _i64, _p]),
+}
+
+_lib = None
+
+
+class UnitairAmdError(RuntimeError):
+    """A call into libunitair_amd.so failed."""
+
+
+def library_path() -> Path:
+    return Path(os.environ.get(LIB_ENV, str(_DEFAULT_PATH)))
+
+
+def load() -> ctypes.CDLL:
+    """Open the library once and bind every declared entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not path.exists():
+        raise ImportError(
+            f"{path} not found: build it with `python -m unitair_amd._build` "
+            f"(or __graft_entry__.build()); unitair_amd has no non-HIP fallback")
+    lib = ctypes.CDLL(str(path))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.ua_abi_version() != ABI_VERSION:
+        raise ImportError(f"{path}: ABI version {lib.ua_abi_version()} != {ABI_VERSION}")
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    """Raise the Python exception matching a library status code."""
+    if rc == UA_OK:
+        return
+    msg = load().ua_last_error().decode(errors="replace")
+    if rc == UA_ERR_INVALID_ARGUMENT:
+        raise ValueError(f"{what}: {msg}")
+    if rc == UA_ERR_UNSUPPORTED:
+        raise NotImplementedError(f"{what}: {msg}")
+    raise UnitairAmdError(f"{what}: {msg}")
+
+
+def stream_handle(device: torch.device) -> ctypes.c_void_p:
+    """hipStream_t of torch's current stream on `device`."""
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def int_array(values) -> ctypes.Array:
+    values = list(values)
+    return (ctypes.c_int * max(1, len(values)))(*values)
+
+
+def i64_array(values) -> ctypes.Array:
+    values = list(values)
+    return (ctypes.c_int64 * max(1, len(values)))(*values)

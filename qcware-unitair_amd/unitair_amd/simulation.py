@@ -1,0 +1,315 @@
+"""Gate application on ROCm GPUs (mirror of ``unitair.simulation``'s hot path).
+
+``apply_operator``, ``act_first_qubits``, ``apply_phase`` and
+``apply_all_qubits`` keep the reference's names, argument meaning, qubit
+ordering, batch broadcasting, validation order and exception types
+(unitair/simulation/operations.py:15-413).  Validation and broadcast
+analysis run here on the host; the arithmetic runs in libunitair_amd.so.
+Inputs are never modified and every result is a new contiguous tensor.
+
+Tensors must be on a ROCm GPU.  There is no CPU path and no fallback: a
+missing library raises ImportError on first use.
+"""
+from __future__ import annotations
+
+import math
+from typing import Iterable, List, Sequence, Tuple
+
+import torch
+
+from . import _lib
+from ._device import on_device, prepare, require_gpu, same_device
+from .states import count_qubits, count_qubits_gate_matrix
+
+__all__ = ["apply_operator", "act_first_qubits", "apply_phase", "apply_all_qubits"]
+
+_SUPPORTED = tuple(_lib.DTYPE_CODES)
+
+
+# ------------------------------------------------------------------ helpers --
+
+def _flat_batch(t: torch.Tensor, inner: int):
+    """View `t` as (rows, inner) with contiguous rows at one element stride;
+    returns (tensor, row stride).  Copies when no single stride fits."""
+    rows = t.numel() // inner if inner else 0
+    v = t.reshape(rows, inner)
+    if inner > 1 and v.stride(1) != 1:
+        v = v.contiguous()
+    return v, (v.stride(0) if rows > 1 else inner)
+
+
+def _check_operator(operator: torch.Tensor, state: torch.Tensor, what: str):
+    if operator.dim() < 2:
+        raise RuntimeError(
+            f"Gate with size {tuple(operator.size())} is incorrectly shaped for an "
+            f"operator batch. Expected size is\n  (*optional_batch_dims, 2^k, 2^k)\n"
+            f"with k the number of qubits on which the gate acts.")
+    if operator.shape[-2] != operator.shape[-1]:
+        raise RuntimeError(f"{what}: operator matrices must be square, got size "
+                           f"{tuple(operator.size())}")
+    require_gpu(state, what)
+    same_device(operator, state, what)
+    if operator.dtype != state.dtype:
+        raise RuntimeError(f"{what}: expected scalar type {state.dtype} for the operator "
+                           f"but found {operator.dtype}")
+    if state.dtype not in _SUPPORTED:
+        raise TypeError(f"{what}: dtype {state.dtype} is not supported "
+                        f"(float32, float64, complex64, complex128)")
+
+
+def _distinct(qubits: List[int]) -> None:
+    seen = set()
+    for q in qubits:
+        if q in seen:
+            # the reference fails inside list.index (unitair/utils.py:26)
+            raise ValueError(f"{q} is not in list")
+        seen.add(q)
+
+
+def _plan_batch(op_batch: Tuple[int, ...], state_batch: Tuple[int, ...], n: int,
+                k: int, qubits: List[int]):
+    """Batch structure of one application, following act_first_qubits_tensor
+    (operations.py:295-329): the op batch dims are right-aligned, as torch.einsum
+    aligns ellipsis dims, against (non-target qubit dims..., state batch dims).
+
+    Returns (out_batch, expand_state, selectors, n_sel_dims, op_batch_part,
+    op_batch) where `op_batch` is the op batch without leading size-1 dims
+    that reach past every ellipsis dim, its first `n_sel_dims` dims line up
+    with non-target qubits, `selectors` lists the (op_dim, qubit) pairs among
+    them of size 2 (the operator is chosen by that qubit's value), and
+    `op_batch_part` is the rest, aligned with the state batch."""
+    expand_state = len(op_batch) > 0 and len(state_batch) == 0
+    sb = tuple(op_batch) if expand_state else tuple(state_batch)
+    remaining = [q for q in range(n) if q not in set(qubits)]
+    ell = [2] * (n - k) + list(sb)            # state's ellipsis dims
+    ob = list(op_batch)
+    if len(ob) > len(ell):
+        extra, ob = ob[:len(ob) - len(ell)], ob[len(ob) - len(ell):]
+        if any(d != 1 for d in extra):
+            raise RuntimeError(
+                f"operator batch {tuple(op_batch)} does not broadcast against state "
+                f"batch {tuple(state_batch)}")
+    lead = len(ell) - len(ob)                 # first ellipsis dim the op covers
+    selectors = []
+    for i, d in enumerate(ob):
+        e = ell[lead + i]
+        if d != 1 and d != e:
+            raise RuntimeError(
+                f"einsum(): operator batch dim of size {d} does not match size {e} "
+                f"(operator batch {tuple(op_batch)}, state batch {tuple(state_batch)})")
+        pos = lead + i
+        if pos < n - k and d == 2:
+            selectors.append((i, remaining[pos]))
+    n_sel_dims = max(0, (n - k) - lead)
+    return sb, expand_state, selectors, n_sel_dims, tuple(ob[n_sel_dims:]), tuple(ob)
+
+
+def _multiplexed(op: torch.Tensor, n_sel_dims: int, sel_dims: List[int], k: int):
+    """Block-diagonal operator on (selector qubits + targets) equivalent to
+    choosing op[s] by the selector qubits' values."""
+    D = 2 ** k
+    # drop size-1 selector dims, keep the size-2 ones in order
+    keep = [slice(None) if i in sel_dims else 0 for i in range(n_sel_dims)]
+    sub = op[tuple(keep)] if n_sel_dims else op
+    m = len(sel_dims)
+    rest = sub.shape[m:-2]
+    sub = sub.reshape((2 ** m,) + tuple(rest) + (D, D))
+    big = torch.zeros(tuple(rest) + (2 ** m, D, 2 ** m, D), dtype=op.dtype, device=op.device)
+    for s in range(2 ** m):
+        big[..., s, :, s, :] = sub[s]
+    return big.reshape(tuple(rest) + (2 ** m * D, 2 ** m * D))
+
+
+def _op_for_batch(op: torch.Tensor, op_batch: Tuple[int, ...], sb: Tuple[int, ...],
+                  D: int):
+    """Returns (op tensor, op batch stride in elements) for a flat batch of
+    prod(sb) entries."""
+    if len(op_batch) == 0 or math.prod(op_batch) == 1 and all(d == 1 for d in op_batch):
+        o = op.reshape(D, D)
+        if not o.is_contiguous():
+            o = o.contiguous()
+        return o, 0
+    if tuple(op_batch) == tuple(sb):
+        o = op.reshape(-1, D, D)
+        if not o.is_contiguous():
+            o = o.contiguous()
+        return o, D * D
+    o = op.expand(tuple(sb) + (D, D)).contiguous().reshape(-1, D, D)
+    return o, D * D
+
+
+def _launch_operator(op, qubits, state, n, k, out_batch, expand_state, op_bstride):
+    lib = _lib.load()
+    N = 2 ** n
+    if expand_state:
+        s_flat, s_stride = state.reshape(N), 0
+        if not s_flat.is_contiguous():
+            s_flat = s_flat.contiguous()
+    else:
+        s_flat, s_stride = _flat_batch(state, N)
+    batch = math.prod(out_batch)
+    out = torch.empty(tuple(out_batch) + (N,), dtype=state.dtype, device=state.device)
+    if batch == 0:
+        return out
+    q = _lib.int_array(qubits)
+    with on_device(state.device):
+        rc = lib.ua_apply_operator(
+            _lib.ptr(out), _lib.ptr(s_flat), _lib.ptr(op), _lib.DTYPE_CODES[state.dtype],
+            n, k, q, batch, s_stride, op_bstride, _lib.stream_handle(state.device))
+    _lib.check(rc, "apply_operator")
+    return out
+
+
+def _apply_dense(operator: torch.Tensor, qubits: List[int], state: torch.Tensor,
+                 n: int, k: int, what: str) -> torch.Tensor:
+    _check_operator(operator, state, what)
+    operator, state = prepare(operator), prepare(state)
+    sb, expand_state, selectors, n_sel, op_batch, ob = _plan_batch(
+        tuple(operator.shape[:-2]), tuple(state.shape[:-1]), n, k, qubits)
+    operator = operator.reshape(ob + tuple(operator.shape[-2:]))
+    if n_sel:
+        sel_dims = [i for i, _ in selectors]
+        operator = _multiplexed(operator, n_sel, sel_dims, k)
+        qubits = [q for _, q in selectors] + list(qubits)
+        k = len(qubits)
+    D = 2 ** k
+    op, op_bstride = _op_for_batch(operator, op_batch, sb, D)
+    return _launch_operator(op, qubits, state, n, k, sb, expand_state, op_bstride)
+
+
+# --------------------------------------------------------------- the surface --
+
+def apply_operator(operator: torch.Tensor, qubits: Iterable[int], state: torch.Tensor):
+    """Apply `operator` to `qubits` of a vector-layout state (operations.py:45-148).
+
+    operator: (*op_batch, 2^k, 2^k); qubits: k distinct qubits, the first one
+    is the most significant bit of the gate index; state: (*state_batch, 2^n).
+    Batching: identical batch dims pair up entry by entry; an unbatched
+    operator acts on every state; a batched operator on an unbatched state
+    yields one output per operator.  More generally the op batch is
+    right-aligned against (non-target qubit dims, state batch dims) exactly
+    as the reference's einsum aligns it.
+    """
+    n = count_qubits(state)
+    qubits = list(qubits)
+    if not set(qubits).issubset(range(n)):
+        raise ValueError(f"qubits={qubits} is not consistent with state vector with "
+                         f"{n} qubits.")
+    k = count_qubits_gate_matrix(operator)
+    if len(qubits) != k:
+        raise ValueError(f"Cannot apply operator with {k} to the {len(qubits)} "
+                         f"qubit sequence {qubits}.")
+    _distinct(qubits)
+    return _apply_dense(operator, [int(q) for q in qubits], state, n, k, "apply_operator")
+
+
+def act_first_qubits(operator: torch.Tensor, state: torch.Tensor):
+    """Apply a k-qubit operator to qubits 0..k-1 (operations.py:236-255)."""
+    n = count_qubits(state)
+    k = count_qubits_gate_matrix(operator)
+    if n < k:
+        raise ValueError(f"Attempted to apply a {k}-qubit gate to {n} qubit(s).")
+    return _apply_dense(operator, list(range(k)), state, n, k, "act_first_qubits")
+
+
+def apply_all_qubits(operator: torch.Tensor, state: torch.Tensor) -> torch.Tensor:
+    """Apply one 2x2 operator to every qubit (operations.py:332-413)."""
+    k = count_qubits_gate_matrix(operator)
+    if k != 1:
+        raise ValueError(f"Expected operator on 1 qubit, found a {k} qubit operator.")
+    n = count_qubits(state)
+    _check_operator(operator, state, "apply_all_qubits")
+    operator, state = prepare(operator), prepare(state)
+    obatch, sbatch = tuple(operator.shape[:-2]), tuple(state.shape[:-1])
+    sb, expand_state, selectors, n_sel, op_batch, _ = _plan_batch(obatch, sbatch, n, 1, [0])
+    if selectors:
+        # The reference lines such dims up with qubit dims of a state whose
+        # qubits it keeps swapping; this build rejects them.
+        raise ValueError(
+            f"apply_all_qubits: operator batch {obatch} has dims beyond the state "
+            f"batch {sbatch} that would be matched against qubit dims")
+    operator = operator.reshape(op_batch + (2, 2))
+    op, op_bstride = _op_for_batch(operator, op_batch, sb, 2)
+    lib = _lib.load()
+    N = 2 ** n
+    if expand_state:
+        s_flat, s_stride = state.reshape(N).contiguous(), 0
+    else:
+        s_flat, s_stride = _flat_batch(state, N)
+    batch = math.prod(sb)
+    out = torch.empty(tuple(sb) + (N,), dtype=state.dtype, device=state.device)
+    if batch == 0:
+        return out
+    with on_device(state.device):
+        rc = lib.ua_apply_all_qubits(
+            _lib.ptr(out), _lib.ptr(s_flat), _lib.ptr(op), _lib.DTYPE_CODES[state.dtype],
+            n, batch, s_stride, op_bstride, _lib.stream_handle(state.device))
+    _lib.check(rc, "apply_all_qubits")
+    return out
+
+
+# ------------------------------------------------------------------- phase --
+
+_PHASE_OF = {torch.float32: torch.complex64, torch.float64: torch.complex128,
+             torch.complex64: torch.complex64, torch.complex128: torch.complex128}
+
+
+def _coalesce(shape: Sequence[int], *strides: Sequence[int]):
+    """Merge neighbouring dims that every operand walks contiguously."""
+    dims = [(s, [st[i] for st in strides]) for i, s in enumerate(shape) if s != 1]
+    if not dims:
+        return [1], [[0] for _ in strides]
+    out = [dims[0]]
+    for size, st in dims[1:]:
+        psize, pst = out[-1]
+        if all(p == s * size for p, s in zip(pst, st)):
+            out[-1] = (psize * size, st)
+        else:
+            out.append((size, st))
+    shape2 = [s for s, _ in out]
+    strides2 = [[st[j] for _, st in out] for j in range(len(strides))]
+    return shape2, strides2
+
+
+def apply_phase(angles: torch.Tensor, state: torch.Tensor) -> torch.Tensor:
+    """exp(-1j * angles) * state with torch broadcasting (operations.py:15-42).
+
+    The phase factor has the precision of `angles` (complex64 for float32
+    angles, even against a complex128 state, as in the reference); the product
+    is formed in the promoted result dtype.
+    """
+    if not isinstance(angles, torch.Tensor):
+        raise TypeError(f"exp(): argument 'input' must be Tensor, not {type(angles).__name__}")
+    require_gpu(state, "apply_phase")
+    if angles.device != state.device:
+        if angles.dim() == 0 and angles.device.type == "cpu":
+            angles = angles.to(state.device)
+        else:
+            same_device(angles, state, "apply_phase")
+    angles, state = prepare(angles), prepare(state)
+    if angles.dtype in (torch.float16, torch.bfloat16, torch.complex32) or \
+            state.dtype in (torch.float16, torch.bfloat16, torch.complex32):
+        raise TypeError("apply_phase: half-precision inputs are not supported")
+    if angles.dtype not in _PHASE_OF:
+        angles = angles.to(torch.float32)      # ints/bool: exp(-1j*x) is complex64
+    if state.dtype not in _PHASE_OF:
+        state = state.to(torch.float32)
+    out_dtype = torch.promote_types(_PHASE_OF[angles.dtype], state.dtype)
+    shape = torch.broadcast_shapes(angles.shape, state.shape)
+    out = torch.empty(shape, dtype=out_dtype, device=state.device)
+    if out.numel() == 0:
+        return out
+    ea, es = angles.expand(shape), state.expand(shape)
+    dims, (sst, ast) = _coalesce(shape, es.stride(), ea.stride())
+    if len(dims) > 8:
+        es, ea = es.contiguous(), ea.contiguous()
+        dims, (sst, ast) = _coalesce(shape, es.stride(), ea.stride())
+    lib = _lib.load()
+    code = _lib.DTYPE_CODES
+    with on_device(state.device):
+        rc = lib.ua_apply_phase(
+            _lib.ptr(out), code[out_dtype], _lib.ptr(es), code[es.dtype], _lib.ptr(ea),
+            code[ea.dtype], len(dims), _lib.i64_array(dims), _lib.i64_array(sst),
+            _lib.i64_array(ast), _lib.stream_handle(state.device))
+    _lib.check(rc, "apply_phase")
+    return out

@@ -1,0 +1,316 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bar: complex128/float64 within 1e-12 and complex64/float32 within 1e-5 of
+the oracle, relative to the largest reference amplitude; CZ/CNOT and the
+batching/layout handling bit-exact where the arithmetic is exact.
+"""
+import functools
+import itertools
+import zlib
+
+import numpy as np
+import pytest
+import torch
+
+import unitair_amd as ua
+from unitair_amd import simulation as sim
+from oracle import unitair_np as U
+
+pytestmark = pytest.mark.gpu
+
+TOL = {torch.complex64: 1e-5, torch.float32: 1e-5, torch.complex128: 1e-12, torch.float64: 1e-12}
+NP_OF = {torch.complex64: np.complex128, torch.complex128: np.complex128,
+         torch.float32: np.float64, torch.float64: np.float64}
+
+
+def rand(shape, dtype, rng):
+    x = rng.standard_normal(shape)
+    if dtype.is_complex:
+        x = x + 1j * rng.standard_normal(shape)
+    return x
+
+
+def rel_err(out, ref):
+    out = np.asarray(out, dtype=np.complex128)
+    ref = np.asarray(ref, dtype=np.complex128)
+    assert out.shape == ref.shape, (out.shape, ref.shape)
+    scale = max(1.0, float(np.max(np.abs(ref)))) if ref.size else 1.0
+    return float(np.max(np.abs(out - ref))) / scale if ref.size else 0.0
+
+
+def to_dev(x, dtype, device):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device=device, dtype=dtype)
+
+
+def cast_np(x, dtype):
+    """The values the GPU sees (rounded to dtype), as float64/complex128."""
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dtype).numpy().astype(NP_OF[dtype])
+
+
+# --------------------------------------------------------- apply_operator --
+
+CASES = []
+for dt in (torch.complex64, torch.complex128, torch.float32, torch.float64):
+    for n, k in [(1, 1), (2, 1), (2, 2), (3, 1), (4, 3), (6, 2), (7, 4), (8, 5), (9, 1),
+                 (10, 2), (11, 3), (12, 5), (6, 0), (7, 6), (8, 7), (5, 5), (13, 1)]:
+        CASES.append((dt, n, k))
+
+
+@pytest.mark.parametrize("dt,n,k", CASES)
+def test_apply_operator_matches_oracle(device, dt, n, k):
+    rng = np.random.default_rng(zlib.crc32(f"{dt}-{n}-{k}".encode()))
+    batches = [((), ()), ((), (3,)), ((2, 3), (2, 3)), ((4,), ()), ((5,), (2, 5))]
+    for ob, sb in batches:
+        qubits = [int(q) for q in rng.permutation(n)[:k]]
+        op = cast_np(rand(ob + (2 ** k, 2 ** k), dt, rng) / np.sqrt(2 ** k), dt)
+        st = cast_np(rand(sb + (2 ** n,), dt, rng) / np.sqrt(2 ** n), dt)
+        out = sim.apply_operator(to_dev(op, dt, device), qubits, to_dev(st, dt, device))
+        ref = U.apply_operator(op, qubits, st)
+        assert out.dtype == dt
+        assert rel_err(out.cpu().numpy(), ref) < TOL[dt], (ob, sb, qubits)
+
+
+def test_apply_operator_all_target_orders(device):
+    rng = np.random.default_rng(5)
+    n = 6
+    st = rand((2, 2 ** n), torch.complex128, rng)
+    for k in (1, 2, 3):
+        for qubits in itertools.permutations(range(n), k):
+            op = rand((2 ** k, 2 ** k), torch.complex128, rng)
+            out = sim.apply_operator(to_dev(op, torch.complex128, device), qubits,
+                                     to_dev(st, torch.complex128, device))
+            assert rel_err(out.cpu().numpy(), U.apply_operator(op, qubits, st)) < 1e-12, qubits
+
+
+def test_apply_operator_qubit_aligned_operator_batch(device):
+    # op batch (2, 4) vs state batch (4,): the reference's einsum matches the
+    # extra op dim with the last non-target qubit
+    rng = np.random.default_rng(9)
+    op = rand((2, 4, 2, 2), torch.complex128, rng)
+    st = rand((4, 8), torch.complex128, rng)
+    for qubits in ([0], [2], [1]):
+        out = sim.apply_operator(to_dev(op, torch.complex128, device), qubits,
+                                 to_dev(st, torch.complex128, device))
+        assert rel_err(out.cpu().numpy(), U.apply_operator(op, qubits, st)) < 1e-12
+
+
+def test_apply_operator_views_and_conj(device):
+    rng = np.random.default_rng(11)
+    dt = torch.complex64
+    big = to_dev(rand((5, 2 ** 8), dt, rng), dt, device)
+    op = to_dev(rand((4, 4), dt, rng), dt, device)
+    # odd element offset (unaligned for 16 B access), strided batch, conj view
+    views = [big[1:4], big[::2], big.view(-1)[1:1 + 2 ** 8], big.t().contiguous().t()[:, :]]
+    for v in views:
+        out = sim.apply_operator(op.conj(), (3, 6), v)
+        ref = U.apply_operator(np.conj(op.cpu().numpy()).astype(np.complex128), (3, 6),
+                               v.cpu().numpy().astype(np.complex128))
+        assert rel_err(out.cpu().numpy(), ref) < 1e-5
+    # input unchanged
+    assert big.is_contiguous()
+
+
+def test_act_first_qubits(device):
+    rng = np.random.default_rng(3)
+    for n, k in [(3, 1), (6, 4), (9, 5)]:
+        op = rand((3, 2 ** k, 2 ** k), torch.complex128, rng)
+        st = rand((3, 2 ** n), torch.complex128, rng)
+        a = sim.act_first_qubits(to_dev(op, torch.complex128, device), to_dev(st, torch.complex128, device))
+        b = sim.apply_operator(to_dev(op, torch.complex128, device), range(k),
+                               to_dev(st, torch.complex128, device))
+        assert torch.equal(a, b)
+        assert rel_err(a.cpu().numpy(), U.act_first_qubits(op, st)) < 1e-12
+
+
+# --------------------------------------------------------- apply_all_qubits --
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128, torch.float64])
+def test_apply_all_qubits(device, dt):
+    rng = np.random.default_rng(21)
+    for n in (1, 2, 5, 9, 12):
+        for ob, sb in [((), ()), ((), (3,)), ((3,), (3,)), ((4,), ())]:
+            op = cast_np(rand(ob + (2, 2), dt, rng) / np.sqrt(2), dt)
+            st = cast_np(rand(sb + (2 ** n,), dt, rng) / np.sqrt(2 ** n), dt)
+            out = sim.apply_all_qubits(to_dev(op, dt, device), to_dev(st, dt, device))
+            assert rel_err(out.cpu().numpy(), U.apply_all_qubits(op, st)) < TOL[dt] * n
+
+
+# ------------------------------------------------------------------- phase --
+
+PHASE_CASES = [
+    # (angle shape, state shape, angle dtype, state dtype)
+    ((2 ** 10,), (2 ** 10,), torch.float32, torch.complex64),
+    ((2 ** 8,), (3, 2 ** 8), torch.float32, torch.complex64),
+    ((3, 2 ** 8), (3, 2 ** 8), torch.float64, torch.complex128),
+    ((3, 1), (3, 2 ** 8), torch.float64, torch.complex128),
+    ((), (4, 2 ** 5), torch.float32, torch.complex64),
+    ((4, 2, 8), (4, 2, 8), torch.float32, torch.float32),
+    ((2 ** 6,), (2, 2 ** 6), torch.float32, torch.complex128),
+    ((2 ** 6,), (2, 2 ** 6), torch.float64, torch.complex64),
+    ((2, 1, 16), (2, 3, 16), torch.complex64, torch.complex64),
+    ((5, 1, 16), (1, 3, 16), torch.complex128, torch.complex128),
+]
+
+
+@pytest.mark.parametrize("ashape,sshape,adt,sdt", PHASE_CASES)
+def test_apply_phase(device, ashape, sshape, adt, sdt):
+    rng = np.random.default_rng(len(ashape) * 7 + len(sshape))
+    ang = cast_np(rand(ashape, adt, rng) * 3, adt)
+    if adt.is_complex:
+        ang = ang.real + 0.1j * ang.imag
+    st = cast_np(rand(sshape, sdt, rng), sdt)
+    out = sim.apply_phase(to_dev(ang, adt, device), to_dev(st, sdt, device))
+    expect_dt = torch.promote_types({torch.float32: torch.complex64,
+                                     torch.float64: torch.complex128}.get(adt, adt), sdt)
+    assert out.dtype == expect_dt
+    a_np = ang.astype(np.float32 if adt == torch.float32 else
+                      np.complex64 if adt == torch.complex64 else ang.dtype)
+    ref = U.apply_phase(a_np, st)
+    tol = 1e-12 if (adt in (torch.float64, torch.complex128) and expect_dt == torch.complex128) else 1e-5
+    assert rel_err(out.cpu().numpy(), ref) < tol
+
+
+def test_apply_phase_cpu_scalar_angle_and_strided_state(device):
+    st = torch.randn(2 ** 7, 4, dtype=torch.complex64, device=device).t()   # non-contiguous
+    out = sim.apply_phase(torch.tensor(0.7), st)
+    ref = U.apply_phase(np.float32(0.7), st.cpu().numpy().astype(np.complex128))
+    assert rel_err(out.cpu().numpy(), ref) < 1e-5
+
+
+def test_apply_phase_round_trip(device):
+    # reference tests/test_operations.py:23-40, float32 angles over a wide range
+    st = torch.randn(3, 2 ** 9, dtype=torch.complex64, device=device)
+    ang = (torch.rand(3, 2 ** 9, device=device) - 0.5) * 2e4
+    back = sim.apply_phase(-ang, sim.apply_phase(ang, st))
+    assert torch.allclose(back, st, atol=1e-4)
+
+
+def test_apply_phase_matches_reference_closed_form(device):
+    # reference tests/test_unitary.py:10-36
+    st = torch.randn(2 ** 5, dtype=torch.complex64, device=device)
+    ang = torch.rand(2 ** 5, device=device)
+    c, s = ang.cos(), ang.sin()
+    basic = torch.complex(c * st.real + s * st.imag, -s * st.real + c * st.imag)
+    assert torch.allclose(sim.apply_phase(ang, st), basic, atol=1e-6)
+    batch = torch.rand(4, 2, 2 ** 3, device=device)
+    angs = torch.rand(4, 2, 2 ** 3, device=device)
+    rot = sim.apply_phase(angs, batch)
+    assert rot.shape == (4, 2, 8)
+    assert torch.equal(rot[2, 0], sim.apply_phase(angs[2, 0], batch[2, 0]))
+
+
+# -------------------------------------------------------------- reductions --
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128, torch.float32])
+def test_reductions(device, dt):
+    rng = np.random.default_rng(4)
+    for shape in [(2 ** 12,), (3, 2 ** 10), (2, 3, 100), (1, 2 ** 17 + 3)]:
+        a = cast_np(rand(shape, dt, rng), dt)
+        b = cast_np(rand(shape, dt, rng), dt)
+        d = rng.standard_normal(shape[-1:])
+        A, B = to_dev(a, dt, device), to_dev(b, dt, device)
+        tol = 1e-12 if dt == torch.complex128 else 1e-6
+        n = ua.norm_squared(A)
+        assert n.shape == shape[:-1] and not n.dtype.is_complex
+        assert np.allclose(n.cpu().numpy(), U.norm_squared(a), rtol=tol, atol=0)
+        assert np.allclose(ua.states.norm_squared_f64(A).cpu().numpy(), U.norm_squared(a),
+                           rtol=1e-13, atol=0)
+        ip = ua.inner_product(A, B).cpu().numpy()
+        assert np.allclose(ip, U.inner_product(a, b), rtol=tol, atol=tol * np.sqrt(shape[-1]))
+        dv = ua.diag_expectation_value(torch.from_numpy(d).to(device), A).cpu().numpy()
+        assert np.allclose(dv, U.diag_expectation_value(d, a), rtol=1e-10,
+                           atol=1e-12 * np.sqrt(shape[-1]))
+        assert np.allclose(ua.abs_squared(A).cpu().numpy(), U.abs_squared(a), rtol=tol * 10)
+
+
+# ----------------------------------------------------- CZ / CNOT in place --
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
+def test_cz_cnot_bit_exact(device, dt):
+    from unitair_amd import _lib, gates
+    lib = _lib.load()
+    n = 9
+    st = torch.randn(3, 2 ** n, dtype=dt, device=device)
+    for a, b in [(0, 8), (8, 0), (3, 7), (7, 3), (1, 2), (8, 7)]:
+        for kind, mat, fn in (("cz", gates.cz(device=device, dtype=dt), lib.ua_apply_cz),
+                              ("cnot", gates.cnot(device=device, dtype=dt), lib.ua_apply_cnot)):
+            dense = sim.apply_operator(mat, (a, b), st)
+            x = st.clone()
+            rc = fn(_lib.ptr(x), _lib.DTYPE_CODES[dt], n, a, b, 3, _lib.stream_handle(device))
+            _lib.check(rc, kind)
+            assert torch.equal(x, dense), (kind, a, b)
+
+
+# ----------------------------------------------------------------- circuits --
+
+def test_layered_12q_batch8_c128(device):
+    """12 qubits, batch (8,), complex128, depth 20: elementwise vs the oracle,
+    through the drop-in calls and through both CircuitRunner forms."""
+    from unitair_amd.workloads import layered_circuit_12q
+    from unitair_amd.circuits import CircuitRunner
+    from unitair_amd import gates as G
+    from unitair_amd.initializations import seeded_state_np
+    from oracle import circuits as oc
+    circ = layered_circuit_12q(seed=1)
+    psi = seeded_state_np(12, (8,), seed=11)
+    ref = np.stack([oc.run_closed_form(circ.gates, p) for p in psi])
+    ref2 = oc.run_restatement(circ.gates[:40], psi)
+    assert rel_err(np.stack([oc.run_closed_form(circ.gates[:40], p) for p in psi]), ref2) < 1e-13
+    dt = torch.complex128
+    x = torch.from_numpy(psi).to(device)
+    for g in circ.gates:                                   # the drop-in loop
+        if g.kind == "phase":
+            x = sim.apply_phase(torch.from_numpy(g.angles).to(device), x)
+        else:
+            x = sim.apply_operator(torch.from_numpy(g.dense_matrix()).to(device), g.qubits, x)
+    assert rel_err(x.cpu().numpy(), ref) < 1e-12
+    for in_place in (True, False):
+        r = CircuitRunner(circ, dtype=dt, device=device, in_place=in_place)
+        y = torch.from_numpy(psi).to(device)
+        out = r.run(y)
+        assert rel_err(out.cpu().numpy(), ref) < 1e-12
+
+
+@functools.lru_cache(maxsize=1)
+def _circuit_24q_and_oracle():
+    from unitair_amd.workloads import random_circuit_24q
+    from unitair_amd.initializations import seeded_state_np
+    from oracle import circuits as oc
+    circ = random_circuit_24q(seed=2)
+    psi = seeded_state_np(24, (), seed=12)
+    return circ, psi, oc.run_closed_form(circ.gates, psi)
+
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
+def test_random_circuit_24q(device, dt):
+    """24 qubits, 200 gates: fidelity error and max-abs error vs the complex128
+    oracle (1e-5 for complex64, 1e-12 for complex128)."""
+    from unitair_amd.circuits import CircuitRunner
+    circ, psi, ref = _circuit_24q_and_oracle()
+    r = CircuitRunner(circ, dtype=dt, device=device, in_place=True)
+    x = torch.from_numpy(psi).to(device, dt)
+    out = r.run(x).cpu().numpy()
+    fid = float(U.fidelity_error(ref, out))
+    mae = U.max_abs_error(ref, out)
+    tol = TOL[dt]
+    print(f"24q {dt}: fidelity error {fid:.3e}, max-abs error {mae:.3e}")
+    assert fid < tol and mae < tol
+
+
+def test_norm_and_inverse_at_28q(device):
+    """Size-independent properties at 28 qubits (2 GiB complex64): a unitary
+    keeps the norm and U then U^H returns the input."""
+    from unitair_amd.gates import haar_unitary
+    rng = np.random.default_rng(6)
+    n = 28
+    x = torch.randn(2 ** n, dtype=torch.complex64, device=device)
+    x /= ua.states.norm_squared_f64(x).sqrt().to(torch.float32)
+    x0 = x.clone()
+    for qubits in [(0,), (27,), (3, 25), (26, 1), (0, 13, 27)]:
+        u = haar_unitary(2 ** len(qubits), rng)
+        ut = torch.from_numpy(u).to(device, torch.complex64)
+        y = sim.apply_operator(ut, qubits, x)
+        assert abs(float(ua.states.norm_squared_f64(y)) - 1.0) < 1e-5
+        z = sim.apply_operator(ut.conj().T.contiguous(), qubits, y)
+        assert float((z - x0).abs().max()) < 1e-5
+        del y, z
