@@ -43,29 +43,39 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kind_of(op):
-    kind, qubits, _ = op
-    return f"dense{len(qubits)}q" if kind == "dense" else kind
-
-
 def time_steps(runner, state, steps, warmup, with_events):
+    """Wall time per step (synchronised on both sides) and, with_events, the
+    per-gate kernel durations from events recorded on the launch stream."""
     import torch
+    from unitair_amd._hip import EventPool
     for _ in range(warmup):
         runner.run(state)
+    pools = [EventPool(len(runner.kinds)) for _ in range(steps)] if with_events else None
     torch.cuda.synchronize()
-    events = [] if with_events else None
     t0 = time.perf_counter()
-    for _ in range(steps):
-        runner.run(state, gate_events=events)
+    for i in range(steps):
+        runner.run(state, events=pools[i] if pools else None)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     per_kind = {}
     if with_events:
-        ops = runner._ops
-        for i, (e0, e1) in enumerate(events):
-            k = kind_of(ops[i % len(ops)])
-            per_kind.setdefault(k, []).append(e0.elapsed_time(e1))
+        for pool in pools:
+            for kind, ms in zip(runner.kinds, pool.elapsed_ms()):
+                per_kind.setdefault(kind, []).append(ms)
     return (t1 - t0) / steps, per_kind
+
+
+def time_graph(runner, state, steps, warmup):
+    import torch
+    runner.capture(state)
+    for _ in range(warmup):
+        runner.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        runner.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
 
 
 def cpu_baseline(circ, n, seed, budget_s):
@@ -145,6 +155,8 @@ def main():
     log(f"timing {args.steps} steps (warmup {args.warmup}), in place")
     sec, per_kind = time_steps(runner, state, args.steps, args.warmup, with_events=True)
     value = ngates * amps / sec
+    log(f"in place: {sec * 1e3:.3f} ms/step, {value:.4e} amp-updates/s; per kernel avg us: "
+        + ", ".join(f"{k}={1e3 * sum(v) / len(v):.1f}" for k, v in per_kind.items()))
 
     extra = {}
     if not args.no_out_of_place:
@@ -153,7 +165,15 @@ def main():
         sec2, _ = time_steps(r2, state, max(3, args.steps // 2), 1, with_events=False)
         extra["out_of_place_amp_updates_per_s"] = ngates * amps / sec2
         extra["out_of_place_ms_per_step"] = sec2 * 1e3
+        log(f"out of place: {sec2 * 1e3:.3f} ms/step")
         del r2
+    log("timing graph replay")
+    r3 = CircuitRunner(circ, dtype=torch.complex64, device=dev, in_place=True)
+    sec3 = time_graph(r3, state, args.steps, args.warmup)
+    extra["graph_replay_amp_updates_per_s"] = ngates * amps / sec3
+    extra["graph_replay_ms_per_step"] = sec3 * 1e3
+    log(f"graph replay: {sec3 * 1e3:.3f} ms/step")
+    del r3
 
     kernels = {k: {"launches_per_step": len(v) // args.steps,
                    "avg_us": 1e3 * float(np.mean(v))} for k, v in per_kind.items()}

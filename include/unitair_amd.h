@@ -160,6 +160,41 @@ int ua_apply_cz(void* state, int dtype, int num_qubits, int qubit_a, int qubit_b
 int ua_apply_cnot(void* state, int dtype, int num_qubits, int control, int target,
                   int64_t batch, ua_stream_t stream);
 
+/*
+ * Circuits.  unitair has no circuit object: a circuit is the user's loop of
+ * apply_operator / apply_phase calls (unitair README.rst:11-16).  ua_run_circuit
+ * takes that loop as data and launches every gate from C onto `stream`.
+ */
+#define UA_MAX_GATE_QUBITS 5
+
+enum {
+  UA_GATE_DENSE = 0, /* apply_operator(data, qubits[:num_targets], state) */
+  UA_GATE_PHASE = 1, /* apply_phase(data, state), data = 2**n angles     */
+  UA_GATE_CZ = 2,    /* apply_operator(cz(), qubits[:2], state)          */
+  UA_GATE_CNOT = 3   /* apply_operator(cnot(), qubits[:2], state)        */
+};
+
+typedef struct {
+  int32_t kind;                        /* UA_GATE_* */
+  int32_t num_targets;                 /* dense: k in 1..UA_MAX_GATE_QUBITS */
+  int32_t qubits[UA_MAX_GATE_QUBITS];  /* unitair qubit order (first = gate MSB) */
+  int32_t data_dtype;                  /* phase: dtype code of the angles */
+  const void* data;                    /* device pointer: (2^k)x(2^k) matrix in the
+                                          state dtype, or 2**n angles */
+} ua_gate;
+
+/*
+ * ua_run_circuit -- apply gates[0..num_gates) in order to `batch` contiguous
+ * states of 2**num_qubits elements.  work == NULL: in place on `state`.
+ * Otherwise every gate reads one buffer and writes the other (the work of one
+ * out-of-place apply_operator call per gate); *result_in_work (if not NULL)
+ * tells which buffer holds the result.  events: NULL, or 2*num_gates
+ * hipEvent_t handles recorded on `stream` before and after each gate.
+ */
+int ua_run_circuit(void* state, void* work, int dtype, int num_qubits, int64_t batch,
+                   const ua_gate* gates, int num_gates, void* const* events,
+                   int* result_in_work, ua_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

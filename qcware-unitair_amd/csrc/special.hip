@@ -68,8 +68,10 @@ void launch_special(void* state, SpecialParams p, bool vec, hipStream_t s) {
     special_kernel<T, CNOT, 1><<<grid_for(p.total), kBlock, 0, s>>>(static_cast<T*>(state), p);
 }
 
-int run_special(bool cnot, void* state, int dtype, int n, int qa, int qb, int64_t batch,
-                hipStream_t s) {
+}  // namespace
+
+int apply_special_impl(bool cnot, void* state, int dtype, int n, int qa, int qb,
+                       int64_t batch, hipStream_t s) {
   const size_t es = elem_size(dtype);
   UA_CHECK_ARG(es != 0, "unsupported dtype code %d", dtype);
   UA_CHECK_ARG(n >= 2 && n <= 62, "num_qubits=%d out of range", n);
@@ -102,17 +104,16 @@ int run_special(bool cnot, void* state, int dtype, int n, int qa, int qb, int64_
   return check_launch(cnot ? "cnot_kernel" : "cz_kernel");
 }
 
-}  // namespace
 }  // namespace ua
 
 extern "C" int ua_apply_cz(void* state, int dtype, int num_qubits, int qubit_a, int qubit_b,
                            int64_t batch, ua_stream_t stream) {
-  return ua::run_special(false, state, dtype, num_qubits, qubit_a, qubit_b, batch,
+  return ua::apply_special_impl(false, state, dtype, num_qubits, qubit_a, qubit_b, batch,
                          static_cast<hipStream_t>(stream));
 }
 
 extern "C" int ua_apply_cnot(void* state, int dtype, int num_qubits, int control, int target,
                              int64_t batch, ua_stream_t stream) {
-  return ua::run_special(true, state, dtype, num_qubits, control, target, batch,
+  return ua::apply_special_impl(true, state, dtype, num_qubits, control, target, batch,
                          static_cast<hipStream_t>(stream));
 }

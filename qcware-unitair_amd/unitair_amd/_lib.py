@@ -57,6 +57,24 @@ SIGNATURES = {
     "ua_apply_cnot": (_i32, [_p, _i32, _i32, _i32, _i32, _i64, _p]),
 }
 
+UA_GATE_DENSE = 0
+UA_GATE_PHASE = 1
+UA_GATE_CZ = 2
+UA_GATE_CNOT = 3
+UA_MAX_GATE_QUBITS = 5
+
+
+class Gate(ctypes.Structure):
+    """ua_gate (include/unitair_amd.h)."""
+    _fields_ = [("kind", ctypes.c_int32), ("num_targets", ctypes.c_int32),
+                ("qubits", ctypes.c_int32 * UA_MAX_GATE_QUBITS),
+                ("data_dtype", ctypes.c_int32), ("data", ctypes.c_void_p)]
+
+
+SIGNATURES["ua_run_circuit"] = (_i32, [_p, _p, _i32, _i32, _i64, ctypes.POINTER(Gate), _i32,
+                                       ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_int), _p])
+
 _lib = None
 
 

@@ -75,7 +75,12 @@ _REAL = {torch.complex64: torch.float32, torch.complex128: torch.float64,
 
 
 class CircuitRunner:
-    """Runs a Circuit on a (batch, 2^n) state tensor on one GPU."""
+    """Runs a Circuit on a (batch, 2^n) state tensor on one GPU.
+
+    The gate list is converted once into the C ABI's ua_gate array (device
+    pointers to the uploaded matrices/angles); run() is then a single call of
+    ua_run_circuit, which launches every gate from C on torch's current
+    stream.  capture() records that call into a CUDA/HIP graph for replay."""
 
     def __init__(self, circuit: Circuit, dtype: torch.dtype = torch.complex64,
                  device="cuda", in_place: bool = True):
@@ -85,22 +90,38 @@ class CircuitRunner:
         self.dtype = dtype
         self.code = _lib.DTYPE_CODES[dtype]
         self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         self.in_place = in_place
         self._buffers = []          # keeps uploaded operands alive
-        self._ops = []
         real = _REAL[dtype]
-        for g in circuit.gates:
+        gates = (_lib.Gate * max(1, len(circuit.gates)))()
+        kinds = []
+        for i, g in enumerate(circuit.gates):
+            c = gates[i]
             if g.kind == "dense":
+                if not 1 <= len(g.qubits) <= _lib.UA_MAX_GATE_QUBITS:
+                    raise ValueError(f"dense gates act on 1..{_lib.UA_MAX_GATE_QUBITS} qubits")
                 m = torch.as_tensor(np.ascontiguousarray(g.matrix)).to(self.device, dtype)
                 self._buffers.append(m)
-                self._ops.append(("dense", g.qubits, m))
+                c.kind, c.data = _lib.UA_GATE_DENSE, m.data_ptr()
+                kinds.append(f"dense{len(g.qubits)}q")
             elif g.kind == "phase":
                 a = torch.as_tensor(np.ascontiguousarray(g.angles)).to(self.device, real)
                 self._buffers.append(a)
-                self._ops.append(("phase", g.qubits, a))
+                c.kind, c.data, c.data_dtype = _lib.UA_GATE_PHASE, a.data_ptr(), \
+                    _lib.DTYPE_CODES[real]
+                kinds.append("phase")
             else:
-                self._ops.append((g.kind, g.qubits, None))
+                c.kind = _lib.UA_GATE_CZ if g.kind == "cz" else _lib.UA_GATE_CNOT
+                kinds.append(g.kind)
+            c.num_targets = len(g.qubits)
+            for j, q in enumerate(g.qubits):
+                c.qubits[j] = q
+        self._gates = gates
+        self.kinds = kinds
         self._work = None
+        self._graph = None
 
     def _check_state(self, state: torch.Tensor):
         if state.device != self.device or state.dtype != self.dtype:
@@ -108,54 +129,53 @@ class CircuitRunner:
         if state.shape[-1] != 2 ** self.n or not state.is_contiguous():
             raise ValueError("state must be contiguous (batch, 2**n)")
 
-    def _launch(self, op, src: torch.Tensor, dst: torch.Tensor, batch: int, stream):
-        kind, qubits, t = op
-        lib, n, code = self.lib, self.n, self.code
-        if kind == "dense":
-            rc = lib.ua_apply_operator(_lib.ptr(dst), _lib.ptr(src), _lib.ptr(t), code, n,
-                                       len(qubits), _lib.int_array(qubits), batch,
-                                       2 ** n, 0, stream)
-        elif kind == "phase":
-            N = 2 ** n
-            shape = _lib.i64_array([batch, N])
-            rc = lib.ua_apply_phase(_lib.ptr(dst), code, _lib.ptr(src), code, _lib.ptr(t),
-                                    _lib.DTYPE_CODES[t.dtype], 2, shape,
-                                    _lib.i64_array([N, 1]), _lib.i64_array([0, 1]), stream)
-        else:
-            if dst.data_ptr() != src.data_ptr():
-                dst.copy_(src)
-            fn = lib.ua_apply_cz if kind == "cz" else lib.ua_apply_cnot
-            rc = fn(_lib.ptr(dst), code, n, qubits[0], qubits[1], batch, stream)
-        if rc:
-            _lib.check(rc, f"circuit gate {kind}{qubits}")
-
-    def run(self, state: torch.Tensor, gate_events: Optional[list] = None) -> torch.Tensor:
+    def run(self, state: torch.Tensor, events=None) -> torch.Tensor:
         """Apply every gate.  In place: `state` is updated and returned.  Out of
-        place: `state` is left unchanged and the result is returned (a work
-        buffer owned by the runner is reused between calls).
+        place: `state` is left unchanged and the result (a work buffer owned by
+        the runner, reused between calls) is returned.
 
-        gate_events, if given, receives one (start, end) pair of timing events
-        per gate, recorded on the current stream around each launch."""
+        events: optional _hip.EventPool of len(circuit) gates; each gate is
+        bracketed by its two events on the current stream."""
         self._check_state(state)
         batch = state.numel() // (2 ** self.n)
-        stream = _lib.stream_handle(self.device)
-        cur = torch.cuda.current_stream(self.device)
         if self.in_place:
-            bufs = [state, state]
+            src, work = state, None
         else:
-            if self._work is None or self._work.shape != state.shape:
+            if self._work is None or self._work[0].shape != state.shape:
                 self._work = [torch.empty_like(state), torch.empty_like(state)]
-            bufs = self._work
-        src = state
-        for i, op in enumerate(self._ops):
-            dst = bufs[i % 2]
-            if gate_events is not None:
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record(cur)
-            self._launch(op, src, dst, batch, stream)
-            if gate_events is not None:
-                e1.record(cur)
-                gate_events.append((e0, e1))
-            src = dst
-        return src
+            self._work[0].copy_(state)
+            src, work = self._work
+        flag = ctypes.c_int(0)
+        ev = None if events is None else ctypes.cast(events.handles,
+                                                     ctypes.POINTER(ctypes.c_void_p))
+        rc = self.lib.ua_run_circuit(
+            _lib.ptr(src), None if work is None else _lib.ptr(work), self.code, self.n,
+            batch, self._gates, len(self.circuit.gates), ev, ctypes.byref(flag),
+            _lib.stream_handle(self.device))
+        _lib.check(rc, "ua_run_circuit")
+        if self.in_place:
+            return state
+        return work if flag.value else src
+
+    def capture(self, state: torch.Tensor):
+        """Record run(state) into a graph; replay() re-runs it on the same
+        buffers.  In place only (the state tensor is the graph's operand)."""
+        if not self.in_place:
+            raise ValueError("graph capture is supported for in-place runners")
+        self._check_state(state)
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.run(state)
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self._graph = (g, state)
+        return g
+
+    def replay(self):
+        if self._graph is None:
+            raise RuntimeError("capture() first")
+        self._graph[0].replay()
+        return self._graph[1]

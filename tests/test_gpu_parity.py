@@ -266,9 +266,12 @@ def test_layered_12q_batch8_c128(device):
     assert rel_err(x.cpu().numpy(), ref) < 1e-12
     for in_place in (True, False):
         r = CircuitRunner(circ, dtype=dt, device=device, in_place=in_place)
-        y = torch.from_numpy(psi).to(device)
-        out = r.run(y)
-        assert rel_err(out.cpu().numpy(), ref) < 1e-12
+        for _ in range(2):                      # a runner is reusable
+            y = torch.from_numpy(psi).to(device)
+            out = r.run(y)
+            assert rel_err(out.cpu().numpy(), ref) < 1e-12
+            if not in_place:
+                assert torch.equal(y, torch.from_numpy(psi).to(device))
 
 
 @functools.lru_cache(maxsize=1)
