@@ -51,10 +51,15 @@ def time_steps(runner, state, steps, warmup, with_events):
     for _ in range(warmup):
         runner.run(state)
     pools = [EventPool(len(runner.kinds)) for _ in range(steps)] if with_events else None
+    if pools is None:
+        pools = [None] * steps
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
-        runner.run(state, events=pools[i] if pools else None)
+        if pools[i] is None:
+            runner.run(state)
+        else:
+            runner.run(state, events=pools[i])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     per_kind = {}
@@ -96,7 +101,7 @@ def cpu_baseline(circ, n, seed, budget_s):
     t0 = time.perf_counter()
     for g in circ.gates:
         if g.kind == "phase":
-            a = torch.from_numpy(g.angles.astype(np.float32))
+            a = torch.from_numpy(np.asarray(g.angles, dtype=np.float32))
             ts = time.perf_counter()
             psi = T.apply_phase(a, psi)
         else:
@@ -174,6 +179,20 @@ def main():
     extra["graph_replay_ms_per_step"] = sec3 * 1e3
     log(f"graph replay: {sec3 * 1e3:.3f} ms/step")
     del r3
+
+    log("timing fused passes")
+    from unitair_amd.fusion import FusedProgram
+    prog = FusedProgram(circ, dtype=torch.complex64, device=dev)
+    sec4, _ = time_steps(prog, state, args.steps, args.warmup, with_events=False)
+    sec5 = time_graph(prog, state, args.steps, args.warmup)
+    extra["fused"] = {"passes": prog.num_passes,
+                      "ms_per_step": sec4 * 1e3,
+                      "amp_updates_per_s": ngates * amps / sec4,
+                      "graph_ms_per_step": sec5 * 1e3,
+                      "graph_amp_updates_per_s": ngates * amps / sec5}
+    log(f"fused: {prog.num_passes} passes, {sec4 * 1e3:.3f} ms/step eager, "
+        f"{sec5 * 1e3:.3f} ms/step graph")
+    del prog
 
     kernels = {k: {"launches_per_step": len(v) // args.steps,
                    "avg_us": 1e3 * float(np.mean(v))} for k, v in per_kind.items()}

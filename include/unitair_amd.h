@@ -55,6 +55,16 @@ int ua_abi_version(void);
 /* Message for the last failing call on this thread ("" if none). */
 const char* ua_last_error(void);
 
+/* Launch-shape knobs (no effect on results; process-wide, not thread-safe):
+ * UA_TUNE_UNROLL        amplitude groups each thread moves per launch, 1/2/4,
+ *                       or 0 (default: 1 for 1-qubit, 2 for wider gates); used
+ *                       when one operator serves the whole batch
+ * UA_TUNE_LANE_EXCHANGE 1 (default): 1-qubit gates on index bits 1..6 move
+ *                       contiguous amplitude pairs and exchange partners
+ *                       across lanes; 0: strided pair access. */
+enum { UA_TUNE_UNROLL = 0, UA_TUNE_LANE_EXCHANGE = 1 };
+int ua_set_tuning(int key, int value);
+
 /*
  * ua_apply_operator -- replaces unitair.simulation.apply_operator
  * (unitair/simulation/operations.py:45-148), i.e. its tensor-layout core
@@ -194,6 +204,58 @@ typedef struct {
 int ua_run_circuit(void* state, void* work, int dtype, int num_qubits, int64_t batch,
                    const ua_gate* gates, int num_gates, void* const* events,
                    int* result_in_work, ua_stream_t stream);
+
+/*
+ * Fused passes: several gates per read and write of the state.  The
+ * reference's only fusion is multiplying 1-qubit gates that hit the same qubit
+ * (unitair/gates/matrix_algebra.py:6-51, apply_to_qubits operations.py:
+ * 416-503); a fused pass applies a whole run of gates to each 2^m-amplitude
+ * tile of the state (m = tile_bits <= 12 "local" index bits) while it sits in
+ * LDS.  In place, complex64 or complex128.
+ *
+ * A pass is cut into stages.  In a stage each thread holds the 16 amplitudes
+ * spanned by the stage's 4 tile bits (its "cube"); dense gates address cube
+ * bits 0..3, element-wise gates address state index bits.
+ */
+#define UA_FUSED_MAX_STAGES 32
+
+enum {
+  UA_FOP_DENSE1 = 0, /* a: cube bit; data: 2x2 matrix (state dtype)          */
+  UA_FOP_DENSE2 = 1, /* a: cube bit of gate-index bit 1, b: of bit 0; 4x4     */
+  UA_FOP_CNOT = 2,   /* a: control state index bit, b: target cube bit       */
+  UA_FOP_CZ = 3,     /* a, b: state index bits                               */
+  UA_FOP_DIAG1 = 4,  /* a: state index bit; data: (d0, d1), state dtype      */
+  UA_FOP_PHASE = 5   /* data: 2**n real angles, exp(-1j*angle) per element
+                        (float32 for complex64 states, float64 for complex128) */
+};
+
+#define UA_FUSED_MATRIX_CACHE 1024 /* elements of gate data a pass keeps on chip */
+
+typedef struct {
+  int32_t kind; /* UA_FOP_* */
+  int32_t a, b;
+  int32_t cache_slot;   /* -1, or element offset of this op's data in the pass's
+                           on-chip cache (data_bstride must be 0; the op's 4, 16
+                           or 2 elements must end within UA_FUSED_MATRIX_CACHE) */
+  const void* data;     /* device pointer */
+  int64_t data_bstride; /* elements between the data of consecutive states */
+} ua_fused_op;
+
+typedef struct {
+  int32_t tile_bits;           /* m in 4..12, m <= num_qubits */
+  int32_t local_bits[12];      /* state bit of tile bit i: ascending, [0..4) = 0,1,2,3 */
+  int32_t num_stages;          /* <= UA_FUSED_MAX_STAGES */
+  int32_t needs_index;         /* 1 if any op is CNOT/CZ/DIAG1/PHASE */
+  int32_t stage_bits[UA_FUSED_MAX_STAGES][4]; /* ascending tile bits of each cube */
+  int32_t stage_ops[UA_FUSED_MAX_STAGES + 1]; /* stage s runs ops [stage_ops[s], stage_ops[s+1]) */
+} ua_fused_pass;
+
+/* Run one fused pass on `batch` contiguous states.  `ops` is the device copy
+ * of the pass's ops; `host_ops` (may be NULL) is the same array in host
+ * memory, used only to validate it before launch. */
+int ua_run_fused_pass(void* state, int dtype, int num_qubits, int64_t batch,
+                      const ua_fused_pass* pass, const ua_fused_op* ops,
+                      const ua_fused_op* host_ops, ua_stream_t stream);
 
 #ifdef __cplusplus
 }

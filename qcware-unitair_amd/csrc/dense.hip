@@ -26,6 +26,10 @@
 #include "ua_common.h"
 
 namespace ua {
+
+int g_unroll = 0;      // units per thread for single-operator launches (0 = auto)
+bool g_use_xch = true; // lane-exchange path for 1-qubit gates on index bits 1..6
+
 namespace {
 
 constexpr int kMaxFastK = 5;
@@ -62,18 +66,20 @@ __device__ __forceinline__ uint64_t offset_of(int j, const FastParams& p) {
   return o;
 }
 
-template <typename T, int K, int MODE, bool LDS>
+template <typename T, int K, int MODE, bool LDS, int UNR>
 __global__ __launch_bounds__(kBlock) void dense_fast_kernel(
     T* out, const T* in, const T* __restrict__ op, FastParams p) {
   using A = Num<T>;
   constexpr int D = 1 << K;
-  constexpr int G = (MODE == VEC) ? 2 : 1;  // groups per thread
+  constexpr int G = (MODE == VEC) ? 2 : 1;  // groups per unit
+  // UNR > 1 only with one operator for the whole launch (op_bstride == 0).
+  const int64_t first = int64_t(blockIdx.x) * (kBlock * UNR) + threadIdx.x;
 
   __shared__ T smat[LDS ? D * D : 1];
   if constexpr (LDS) {
-    // The host only picks LDS staging when all threads of a workgroup use
-    // the same operator.
-    const int64_t b0 = (int64_t(blockIdx.x) * kBlock) >> p.log2_tps;
+    // The host only stages in LDS when every thread of the workgroup uses the
+    // same operator.
+    const int64_t b0 = (int64_t(blockIdx.x) * (kBlock * UNR)) >> p.log2_tps;
     const T* opb = op + b0 * p.op_bstride;
     for (int e = threadIdx.x; e < D * D; e += kBlock) {
       const int r = e >> K, c = e & (D - 1);
@@ -82,34 +88,39 @@ __global__ __launch_bounds__(kBlock) void dense_fast_kernel(
     __syncthreads();
   }
 
-  const int64_t tid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (tid >= p.total) return;
-  const int64_t b = tid >> p.log2_tps;
-  const uint64_t t = uint64_t(tid) & ((uint64_t(1) << p.log2_tps) - 1);
-  const uint64_t base = insert_zero_bits(t * G, p.tb, K);
-  const T* src = in + b * p.in_bstride + base;
-  T* dst = out + b * p.out_bstride + base;
-  const T* opb = op + b * p.op_bstride;
-
-  // ---- load the group(s)
-  T a[D][G];
-  if constexpr (MODE == VEC) {
+  // ---- load UNR units (each G groups of D amplitudes) before any math
+  T a[UNR][D][G];
+  T* dst[UNR];
+  const T* opb = op;
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-      const Pair<T> v = *reinterpret_cast<const Pair<T>*>(src + offset_of<K>(j, p));
-      a[j][0] = v.v[0];
-      a[j][1] = v.v[1];
+  for (int u = 0; u < UNR; ++u) {
+    const int64_t unit = first + int64_t(u) * kBlock;
+    dst[u] = nullptr;
+    if (unit >= p.total) continue;
+    const int64_t b = unit >> p.log2_tps;
+    const uint64_t t = uint64_t(unit) & ((uint64_t(1) << p.log2_tps) - 1);
+    const uint64_t base = insert_zero_bits(t * G, p.tb, K);
+    const T* src = in + b * p.in_bstride + base;
+    dst[u] = out + b * p.out_bstride + base;
+    if (UNR == 1) opb = op + b * p.op_bstride;
+    if constexpr (MODE == VEC) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const Pair<T> v = *reinterpret_cast<const Pair<T>*>(src + offset_of<K>(j, p));
+        a[u][j][0] = v.v[0];
+        a[u][j][1] = v.v[1];
+      }
+    } else if constexpr (MODE == PAIR) {
+#pragma unroll
+      for (int j = 0; j < D; j += 2) {
+        const Pair<T> v = *reinterpret_cast<const Pair<T>*>(src + offset_of<K>(j, p));
+        a[u][j][0] = v.v[0];
+        a[u][j + 1][0] = v.v[1];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < D; ++j) a[u][j][0] = src[offset_of<K>(j, p)];
     }
-  } else if constexpr (MODE == PAIR) {
-#pragma unroll
-    for (int j = 0; j < D; j += 2) {
-      const Pair<T> v = *reinterpret_cast<const Pair<T>*>(src + offset_of<K>(j, p));
-      a[j][0] = v.v[0];
-      a[j + 1][0] = v.v[1];
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < D; ++j) a[j][0] = src[offset_of<K>(j, p)];
   }
 
   // ---- matrix element accessor (canonical order)
@@ -132,41 +143,103 @@ __global__ __launch_bounds__(kBlock) void dense_fast_kernel(
   };
 
   // ---- multiply and store
-  if constexpr (MODE == PAIR) {
 #pragma unroll
-    for (int r = 0; r < D; r += 2) {
-      T acc0 = A::zero(), acc1 = A::zero();
+  for (int u = 0; u < UNR; ++u) {
+    if (dst[u] == nullptr) continue;
+    if constexpr (MODE == PAIR) {
 #pragma unroll
-      for (int j = 0; j < D; ++j) {
-        acc0 = A::mac(acc0, U(r, j), a[j][0]);
-        acc1 = A::mac(acc1, U(r + 1, j), a[j][0]);
-      }
-      Pair<T> v;
-      v.v[0] = acc0;
-      v.v[1] = acc1;
-      *reinterpret_cast<Pair<T>*>(dst + offset_of<K>(r, p)) = v;
-    }
-  } else {
+      for (int r = 0; r < D; r += 2) {
+        T acc0 = A::zero(), acc1 = A::zero();
 #pragma unroll
-    for (int r = 0; r < D; ++r) {
-      T acc[G];
-#pragma unroll
-      for (int g = 0; g < G; ++g) acc[g] = A::zero();
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        const T u = U(r, j);
-#pragma unroll
-        for (int g = 0; g < G; ++g) acc[g] = A::mac(acc[g], u, a[j][g]);
-      }
-      if constexpr (MODE == VEC) {
+        for (int j = 0; j < D; ++j) {
+          acc0 = A::mac(acc0, U(r, j), a[u][j][0]);
+          acc1 = A::mac(acc1, U(r + 1, j), a[u][j][0]);
+        }
         Pair<T> v;
-        v.v[0] = acc[0];
-        v.v[1] = acc[1];
-        *reinterpret_cast<Pair<T>*>(dst + offset_of<K>(r, p)) = v;
-      } else {
-        dst[offset_of<K>(r, p)] = acc[0];
+        v.v[0] = acc0;
+        v.v[1] = acc1;
+        *reinterpret_cast<Pair<T>*>(dst[u] + offset_of<K>(r, p)) = v;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        T acc[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g] = A::zero();
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const T uu = U(r, j);
+#pragma unroll
+          for (int g = 0; g < G; ++g) acc[g] = A::mac(acc[g], uu, a[u][j][g]);
+        }
+        if constexpr (MODE == VEC) {
+          Pair<T> v;
+          v.v[0] = acc[0];
+          v.v[1] = acc[1];
+          *reinterpret_cast<Pair<T>*>(dst[u] + offset_of<K>(r, p)) = v;
+        } else {
+          dst[u][offset_of<K>(r, p)] = acc[0];
+        }
       }
     }
+  }
+}
+
+// One-qubit gate whose target is index bit b in [1, 6] (with 2 elements per
+// lane: lane bit b-1).  Every lane moves one contiguous pair of amplitudes,
+// so each wavefront access is one contiguous block; the partner amplitudes
+// differ in lane bit b-1 and arrive by a cross-lane exchange.  The host
+// guarantees 2**n >= 128, so every wavefront is full and inside one state.
+template <typename T> __device__ __forceinline__ T shfl_xor_val(T v, int m);
+template <> __device__ __forceinline__ float shfl_xor_val(float v, int m) { return __shfl_xor(v, m, 64); }
+template <> __device__ __forceinline__ double shfl_xor_val(double v, int m) { return __shfl_xor(v, m, 64); }
+template <> __device__ __forceinline__ float2 shfl_xor_val(float2 v, int m) {
+  return make_float2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+template <> __device__ __forceinline__ double2 shfl_xor_val(double2 v, int m) {
+  return make_double2(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
+struct XchParams {
+  int64_t total;        // units (pairs of elements) over the whole batch
+  int64_t in_bstride;   // elements between input states
+  int64_t out_bstride;  // elements between output states
+  int log2_ups;         // log2(units per state)
+  int lane_mask;        // 1 << (b - 1)
+};
+
+template <typename T, int UNR>
+__global__ __launch_bounds__(kBlock) void dense_xch_kernel(
+    T* out, const T* in, const T* __restrict__ op, XchParams p) {
+  using A = Num<T>;
+  const T u00 = op[0], u01 = op[1], u10 = op[2], u11 = op[3];
+  const int64_t first = int64_t(blockIdx.x) * (kBlock * UNR) + threadIdx.x;
+  Pair<T> v[UNR];
+  T* dst[UNR];
+#pragma unroll
+  for (int u = 0; u < UNR; ++u) {
+    const int64_t unit = first + int64_t(u) * kBlock;
+    dst[u] = nullptr;
+    if (unit >= p.total) continue;   // uniform over a wavefront
+    const int64_t b = unit >> p.log2_ups;
+    const int64_t e = (unit & ((int64_t(1) << p.log2_ups) - 1)) * 2;
+    v[u] = *reinterpret_cast<const Pair<T>*>(in + b * p.in_bstride + e);
+    dst[u] = out + b * p.out_bstride + e;
+  }
+  const bool upper = (threadIdx.x & p.lane_mask) != 0;
+  // row of the gate this lane produces and the coefficients for (own, partner)
+  const T c_own = upper ? u11 : u00;
+  const T c_par = upper ? u10 : u01;
+#pragma unroll
+  for (int u = 0; u < UNR; ++u) {
+    if (dst[u] == nullptr) continue;
+    Pair<T> w;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const T partner = shfl_xor_val(v[u].v[h], p.lane_mask);
+      w.v[h] = A::mac(A::mac(A::zero(), c_own, v[u].v[h]), c_par, partner);
+    }
+    *reinterpret_cast<Pair<T>*>(dst[u]) = w;
   }
 }
 
@@ -207,41 +280,66 @@ __global__ __launch_bounds__(kBlock) void dense_generic_kernel(
   out[tid] = acc;
 }
 
+
 template <typename T, int K, int MODE, bool LDS>
-void launch_fast(void* out, const void* in, const void* op, const FastParams& p,
+void launch_fast(void* out, const void* in, const void* op, const FastParams& p, int unr,
                  hipStream_t s) {
-  dense_fast_kernel<T, K, MODE, LDS><<<grid_for(p.total), kBlock, 0, s>>>(
-      static_cast<T*>(out), static_cast<const T*>(in), static_cast<const T*>(op), p);
+  T* o = static_cast<T*>(out);
+  const T* i = static_cast<const T*>(in);
+  const T* m = static_cast<const T*>(op);
+  switch (unr) {
+    case 4:
+      dense_fast_kernel<T, K, MODE, LDS, 4><<<grid_for(p.total, kBlock * 4), kBlock, 0, s>>>(o, i, m, p);
+      break;
+    case 2:
+      dense_fast_kernel<T, K, MODE, LDS, 2><<<grid_for(p.total, kBlock * 2), kBlock, 0, s>>>(o, i, m, p);
+      break;
+    default:
+      dense_fast_kernel<T, K, MODE, LDS, 1><<<grid_for(p.total, kBlock), kBlock, 0, s>>>(o, i, m, p);
+  }
 }
 
 template <typename T, int K, int MODE>
 void launch_fast_lds(bool lds, void* out, const void* in, const void* op,
-                     const FastParams& p, hipStream_t s) {
+                     const FastParams& p, int unr, hipStream_t s) {
   if constexpr (K >= 3) {
-    if (lds) return launch_fast<T, K, MODE, true>(out, in, op, p, s);
+    if (lds) return launch_fast<T, K, MODE, true>(out, in, op, p, unr, s);
   }
-  launch_fast<T, K, MODE, false>(out, in, op, p, s);
+  launch_fast<T, K, MODE, false>(out, in, op, p, unr, s);
 }
 
 template <typename T, int K>
 void launch_fast_mode(int mode, bool lds, void* out, const void* in,
-                      const void* op, const FastParams& p, hipStream_t s) {
+                      const void* op, const FastParams& p, int unr, hipStream_t s) {
   switch (mode) {
-    case VEC: return launch_fast_lds<T, K, VEC>(lds, out, in, op, p, s);
-    case PAIR: return launch_fast_lds<T, K, PAIR>(lds, out, in, op, p, s);
-    default: return launch_fast_lds<T, K, SCALAR>(lds, out, in, op, p, s);
+    case VEC: return launch_fast_lds<T, K, VEC>(lds, out, in, op, p, unr, s);
+    case PAIR: return launch_fast_lds<T, K, PAIR>(lds, out, in, op, p, unr, s);
+    default: return launch_fast_lds<T, K, SCALAR>(lds, out, in, op, p, unr, s);
   }
 }
 
 template <typename T>
 void launch_fast_k(int k, int mode, bool lds, void* out, const void* in,
-                   const void* op, const FastParams& p, hipStream_t s) {
+                   const void* op, const FastParams& p, int unr, hipStream_t s) {
   switch (k) {
-    case 1: return launch_fast_mode<T, 1>(mode, lds, out, in, op, p, s);
-    case 2: return launch_fast_mode<T, 2>(mode, lds, out, in, op, p, s);
-    case 3: return launch_fast_mode<T, 3>(mode, lds, out, in, op, p, s);
-    case 4: return launch_fast_mode<T, 4>(mode, lds, out, in, op, p, s);
-    case 5: return launch_fast_mode<T, 5>(mode, lds, out, in, op, p, s);
+    case 1: return launch_fast_mode<T, 1>(mode, lds, out, in, op, p, unr, s);
+    case 2: return launch_fast_mode<T, 2>(mode, lds, out, in, op, p, unr, s);
+    case 3: return launch_fast_mode<T, 3>(mode, lds, out, in, op, p, unr, s);
+    case 4: return launch_fast_mode<T, 4>(mode, lds, out, in, op, p, unr, s);
+    case 5: return launch_fast_mode<T, 5>(mode, lds, out, in, op, p, unr, s);
+  }
+}
+
+template <typename T>
+void launch_xch(void* out, const void* in, const void* op, const XchParams& p, int unr,
+                hipStream_t s) {
+  T* o = static_cast<T*>(out);
+  const T* i = static_cast<const T*>(in);
+  const T* m = static_cast<const T*>(op);
+  switch (unr) {
+    case 4: dense_xch_kernel<T, 4><<<grid_for(p.total, kBlock * 4), kBlock, 0, s>>>(o, i, m, p); break;
+    case 2: dense_xch_kernel<T, 2><<<grid_for(p.total, kBlock * 2), kBlock, 0, s>>>(o, i, m, p); break;
+    default: dense_xch_kernel<T, 1><<<grid_for(p.total, kBlock), kBlock, 0, s>>>(o, i, m, p);
   }
 }
 
@@ -297,6 +395,25 @@ int apply_operator_impl(void* out, const void* state, const void* op, int dtype,
     const size_t pair_align = 2 * es;
     const bool io_aligned = aligned(state, state_bstride, es, pair_align) &&
                             aligned(out, N, es, pair_align);
+    // measured on MI355X: 1-qubit gates stream best with one group pair per
+    // thread, wider gates with two units per thread
+    const int unr = (op_bstride != 0) ? 1 : (g_unroll ? g_unroll : (k == 1 ? 1 : 2));
+    if (g_use_xch && k == 1 && io_aligned && op_bstride == 0 && n >= 7 && tb[0] >= 1 &&
+        tb[0] <= 6) {
+      XchParams x{};
+      x.log2_ups = n - 1;
+      x.total = batch << x.log2_ups;
+      x.in_bstride = state_bstride;
+      x.out_bstride = N;
+      x.lane_mask = 1 << (tb[0] - 1);
+      switch (dtype) {
+        case UA_FLOAT32: launch_xch<float>(out, state, op, x, unr, s); break;
+        case UA_FLOAT64: launch_xch<double>(out, state, op, x, unr, s); break;
+        case UA_COMPLEX64: launch_xch<float2>(out, state, op, x, unr, s); break;
+        case UA_COMPLEX128: launch_xch<double2>(out, state, op, x, unr, s); break;
+      }
+      return check_launch("dense_xch_kernel");
+    }
     int mode = SCALAR;
     if (io_aligned) mode = (tb[0] == 0) ? PAIR : VEC;
     const int log2_groups = n - k;  // groups per state
@@ -308,10 +425,10 @@ int apply_operator_impl(void* out, const void* state, const void* op, int dtype,
     p.op_bstride = op_bstride;
     const bool lds = (k >= 3) && (op_bstride == 0 || log2_tps >= 8);
     switch (dtype) {
-      case UA_FLOAT32: launch_fast_k<float>(k, mode, lds, out, state, op, p, s); break;
-      case UA_FLOAT64: launch_fast_k<double>(k, mode, lds, out, state, op, p, s); break;
-      case UA_COMPLEX64: launch_fast_k<float2>(k, mode, lds, out, state, op, p, s); break;
-      case UA_COMPLEX128: launch_fast_k<double2>(k, mode, lds, out, state, op, p, s); break;
+      case UA_FLOAT32: launch_fast_k<float>(k, mode, lds, out, state, op, p, unr, s); break;
+      case UA_FLOAT64: launch_fast_k<double>(k, mode, lds, out, state, op, p, unr, s); break;
+      case UA_COMPLEX64: launch_fast_k<float2>(k, mode, lds, out, state, op, p, unr, s); break;
+      case UA_COMPLEX128: launch_fast_k<double2>(k, mode, lds, out, state, op, p, unr, s); break;
     }
     return check_launch("dense_fast_kernel");
   }
@@ -353,6 +470,21 @@ int apply_operator_impl(void* out, const void* state, const void* op, int dtype,
 }
 
 }  // namespace ua
+
+extern "C" int ua_set_tuning(int key, int value) {
+  switch (key) {
+    case UA_TUNE_UNROLL:
+      UA_CHECK_ARG(value == 0 || value == 1 || value == 2 || value == 4,
+                   "unroll must be 0 (auto), 1, 2 or 4");
+      ua::g_unroll = value;
+      return UA_OK;
+    case UA_TUNE_LANE_EXCHANGE:
+      ua::g_use_xch = value != 0;
+      return UA_OK;
+  }
+  ua::set_error("unknown tuning key %d", key);
+  return UA_ERR_INVALID_ARGUMENT;
+}
 
 extern "C" int ua_apply_operator(void* out, const void* state, const void* op,
                                  int dtype, int num_qubits, int op_num_qubits,

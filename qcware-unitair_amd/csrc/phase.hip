@@ -98,6 +98,35 @@ __global__ __launch_bounds__(kBlock) void phase_kernel(
   }
 }
 
+// CONTIG layout, two neighbouring elements per access and UNR pairs per
+// thread: a full wavefront moves contiguous 16 B (complex64) pieces of state
+// and 8 B pieces of angles.
+template <typename TO, typename TS, typename TA, int UNR>
+__global__ __launch_bounds__(kBlock) void phase_pair_kernel(
+    TO* out, const TS* state, const TA* __restrict__ angles, int64_t pairs) {
+  const int64_t first = int64_t(blockIdx.x) * (kBlock * UNR) + threadIdx.x;
+  Pair<TS> x[UNR];
+  Pair<TA> a[UNR];
+#pragma unroll
+  for (int u = 0; u < UNR; ++u) {
+    const int64_t i = first + int64_t(u) * kBlock;
+    if (i < pairs) {
+      x[u] = reinterpret_cast<const Pair<TS>*>(state)[i];
+      a[u] = reinterpret_cast<const Pair<TA>*>(angles)[i];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < UNR; ++u) {
+    const int64_t i = first + int64_t(u) * kBlock;
+    if (i >= pairs) continue;
+    Pair<TO> y;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      y.v[h] = cmul(Widen<TO>::from(phase_of(a[u].v[h])), Widen<TO>::from(x[u].v[h]));
+    reinterpret_cast<Pair<TO>*>(out)[i] = y;
+  }
+}
+
 template <typename TO, typename TS, typename TA>
 void launch_phase(int mode, void* out, const void* st, const void* an,
                   const PhaseParams& p, hipStream_t s) {
@@ -107,6 +136,15 @@ void launch_phase(int mode, void* out, const void* st, const void* an,
   TO* o = static_cast<TO*>(out);
   const TS* x = static_cast<const TS*>(st);
   const TA* a = static_cast<const TA*>(an);
+  const bool pair_ok = (p.total % 2 == 0) &&
+                       reinterpret_cast<uintptr_t>(out) % (2 * sizeof(TO)) == 0 &&
+                       reinterpret_cast<uintptr_t>(st) % (2 * sizeof(TS)) == 0 &&
+                       reinterpret_cast<uintptr_t>(an) % (2 * sizeof(TA)) == 0;
+  if (mode == CONTIG && pair_ok) {
+    const int64_t pairs = p.total / 2;
+    phase_pair_kernel<TO, TS, TA, 2><<<grid_for(pairs, kBlock * 2), kBlock, 0, s>>>(o, x, a, pairs);
+    return;
+  }
   switch (mode) {
     case CONTIG: phase_kernel<TO, TS, TA, CONTIG><<<g, kBlock, 0, s>>>(o, x, a, p); break;
     case TILE: phase_kernel<TO, TS, TA, TILE><<<g, kBlock, 0, s>>>(o, x, a, p); break;

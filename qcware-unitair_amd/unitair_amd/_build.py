@@ -33,6 +33,10 @@ COMPILE_FLAGS = [
     # packed FP32 instructions stay disabled in every kernel of this library
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-mcode-object-version=5",
+    # keep per-case register arrays in registers: otherwise SimplifyCFG merges
+    # the arms of runtime switches over cube bits into dynamically indexed
+    # stack memory (scratch)
+    "-mllvm", "-simplifycfg-sink-common=false",
 ]
 
 

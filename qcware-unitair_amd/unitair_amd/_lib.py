@@ -45,6 +45,7 @@ _pi64 = ctypes.POINTER(ctypes.c_int64)
 SIGNATURES = {
     "ua_abi_version": (_i32, []),
     "ua_last_error": (ctypes.c_char_p, []),
+    "ua_set_tuning": (_i32, [_i32, _i32]),
     "ua_apply_operator": (_i32, [_p, _p, _p, _i32, _i32, _i32, _pi32, _i64, _i64, _i64, _p]),
     "ua_apply_phase": (_i32, [_p, _i32, _p, _i32, _p, _i32, _i32, _pi64, _pi64, _pi64, _p]),
     "ua_apply_all_qubits": (_i32, [_p, _p, _p, _i32, _i32, _i64, _i64, _i64, _p]),
@@ -71,6 +72,27 @@ class Gate(ctypes.Structure):
                 ("data_dtype", ctypes.c_int32), ("data", ctypes.c_void_p)]
 
 
+UA_FUSED_MAX_STAGES = 32
+
+
+class FusedOp(ctypes.Structure):
+    """ua_fused_op (include/unitair_amd.h)."""
+    _fields_ = [("kind", ctypes.c_int32), ("a", ctypes.c_int32), ("b", ctypes.c_int32),
+                ("cache_slot", ctypes.c_int32), ("data", ctypes.c_void_p),
+                ("data_bstride", ctypes.c_int64)]
+
+
+class FusedPass(ctypes.Structure):
+    """ua_fused_pass (include/unitair_amd.h)."""
+    _fields_ = [("tile_bits", ctypes.c_int32), ("local_bits", ctypes.c_int32 * 12),
+                ("num_stages", ctypes.c_int32), ("needs_index", ctypes.c_int32),
+                ("stage_bits", (ctypes.c_int32 * 4) * UA_FUSED_MAX_STAGES),
+                ("stage_ops", ctypes.c_int32 * (UA_FUSED_MAX_STAGES + 1))]
+
+
+UA_FUSED_MATRIX_CACHE = 1024
+SIGNATURES["ua_run_fused_pass"] = (_i32, [_p, _i32, _i32, _i64, ctypes.POINTER(FusedPass),
+                                          _p, _p, _p])
 SIGNATURES["ua_run_circuit"] = (_i32, [_p, _p, _i32, _i32, _i64, ctypes.POINTER(Gate), _i32,
                                        ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.POINTER(ctypes.c_int), _p])

@@ -1,0 +1,321 @@
+"""Compile a gate list into fused passes (ua_run_fused_pass) and run them.
+
+A pass keeps m = min(n, 12) index bits local: bits 0..3 always (so tiles are
+made of 128 B runs) plus the target bits of the gates it absorbs.  Gates are
+taken in circuit order; a gate joins the current pass while the union of
+local bits stays within m:
+  dense 1-/2-qubit gate : needs its target bits local
+  CNOT                  : needs its target bit local (control: any bit)
+  CZ, phase, diagonal   : need nothing (applied element by element)
+Wider dense gates end the pass and run as their own ua_apply_operator call.
+Inside a pass, gates are grouped into stages whose dense targets fit in a
+4-bit cube.  Consecutive 1-qubit gates on the same qubit are multiplied on
+the host first (complex128), as unitair's fuse_single_qubit_operators
+(unitair/gates/matrix_algebra.py:6-51) does for apply_to_qubits.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .circuits import Circuit, Gate
+
+TILE_BITS = 12
+CUBE_BITS = 4
+MAX_STAGES = _lib.UA_FUSED_MAX_STAGES
+
+FOP_DENSE1, FOP_DENSE2, FOP_CNOT, FOP_CZ, FOP_DIAG1, FOP_PHASE = range(6)
+
+
+FusedOp = _lib.FusedOp
+FusedPass = _lib.FusedPass
+
+
+@dataclass
+class _Item:
+    """A gate in state-bit terms.  kind: dense1/dense2/cnot/cz/phase/diag1."""
+    kind: str
+    bits: tuple                      # state bits (dense: gate MSB first)
+    data: Optional[np.ndarray] = None
+    per_state: bool = False          # data has a leading batch dim
+
+
+@dataclass
+class _Pass:
+    local: List[int]
+    items: List[_Item] = field(default_factory=list)
+
+
+def _merge_1q(items: List[_Item]) -> List[_Item]:
+    """Multiply runs of 1-qubit gates on the same bit that are adjacent in
+    the gate order (nothing in between touches that bit)."""
+    out: List[_Item] = []
+    last_on_bit = {}
+    for it in items:
+        touched = set(it.bits) if it.kind not in ("phase",) else None
+        if it.kind == "dense1" and not it.per_state:
+            j = last_on_bit.get(it.bits[0])
+            if j is not None and out[j].kind == "dense1" and not out[j].per_state:
+                out[j] = _Item("dense1", it.bits, it.data @ out[j].data)
+                continue
+            out.append(it)
+            last_on_bit[it.bits[0]] = len(out) - 1
+            continue
+        out.append(it)
+        if touched is None:          # full-state diagonal touches every bit
+            last_on_bit.clear()
+        else:
+            for bt in touched:
+                last_on_bit.pop(bt, None)
+    return out
+
+
+def _items_from_circuit(circ: Circuit) -> List[_Item]:
+    n = circ.num_qubits
+    items = []
+    for g in circ.gates:
+        bits = tuple(n - 1 - q for q in g.qubits)
+        if g.kind == "dense":
+            k = len(g.qubits)
+            items.append(_Item(f"dense{k}" if k <= 2 else "wide", bits,
+                               np.asarray(g.matrix, dtype=np.complex128)))
+        elif g.kind == "phase":
+            items.append(_Item("phase", (), g.angles))
+        else:
+            items.append(_Item(g.kind, bits))
+    return items
+
+
+def _needs(it: _Item):
+    if it.kind in ("dense1", "dense2"):
+        return set(it.bits)
+    if it.kind == "cnot":
+        return {it.bits[1]}
+    return set()
+
+
+def plan_passes(items: Sequence[_Item], n: int, tile_bits: int = TILE_BITS):
+    """Greedy split of the item list into fused passes and wide gates."""
+    m = min(n, tile_bits)
+    base = set(range(min(n, CUBE_BITS)))
+    out = []
+    cur: Optional[_Pass] = None
+    cur_local = set()
+    for it in items:
+        if it.kind == "wide":
+            if cur is not None:
+                out.append(cur)
+                cur = None
+            out.append(it)
+            continue
+        need = _needs(it)
+        if cur is not None and len(cur_local | need) <= m:
+            cur_local |= need
+            cur.items.append(it)
+            continue
+        if cur is not None:
+            out.append(cur)
+        cur_local = base | need
+        cur = _Pass(local=[], items=[it])
+    if cur is not None:
+        out.append(cur)
+    # pad each pass's local set to m bits and fix its order
+    for p in out:
+        if isinstance(p, _Pass):
+            loc = set(range(min(n, CUBE_BITS)))
+            for it in p.items:
+                loc |= _needs(it)
+            for bt in range(n):
+                if len(loc) >= m:
+                    break
+                loc.add(bt)
+            p.local = sorted(loc)
+    return out
+
+
+def _stages(p: _Pass):
+    """Split a pass into stages: [(cube tile bits (sorted, 4), [items])]."""
+    tile_of = {s: i for i, s in enumerate(p.local)}
+    m = len(p.local)
+    stages = []
+    cube, ops = set(), []
+    for it in p.items:
+        need = {tile_of[b] for b in _needs(it)}
+        if len(cube | need) <= CUBE_BITS:
+            cube |= need
+            ops.append(it)
+        else:
+            stages.append((cube, ops))
+            cube, ops = set(need), [it]
+    stages.append((cube, ops))
+    result = []
+    for cube, ops in stages:
+        cube = set(cube)
+        for t in range(m):
+            if len(cube) >= CUBE_BITS:
+                break
+            cube.add(t)
+        result.append((sorted(cube), ops))
+    return result
+
+
+class FusedProgram:
+    """A circuit compiled into fused passes for one (n, dtype, device)."""
+
+    def __init__(self, circuit: Circuit, dtype: torch.dtype = torch.complex64,
+                 device="cuda", tile_bits: int = TILE_BITS, merge_1q: bool = True):
+        if dtype not in (torch.complex64, torch.complex128):
+            raise ValueError("fused programs run complex64/complex128 states")
+        self.lib = _lib.load()
+        self.n = circuit.num_qubits
+        if self.n < CUBE_BITS:
+            raise ValueError("fused programs need at least 4 qubits")
+        self.dtype = dtype
+        self.code = _lib.DTYPE_CODES[dtype]
+        self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        real = torch.float32 if dtype == torch.complex64 else torch.float64
+        items = _items_from_circuit(circuit)
+        if merge_1q:
+            items = _merge_1q(items)
+        plan = plan_passes(items, self.n, tile_bits)
+        self._keep = []
+        self.steps = []          # ("fused", FusedPass, ops_ptr, nops) | ("wide", qubits, mat)
+        mats, op_structs = [], []
+        pending = []             # (op index, item) whose data pointer is set later
+        for p in plan:
+            if isinstance(p, _Item):
+                m = torch.from_numpy(p.data).to(self.device, dtype)
+                self._keep.append(m)
+                qubits = [self.n - 1 - b for b in p.bits]
+                self.steps.append(("wide", qubits, m))
+                continue
+            stages = _stages(p)
+            for chunk in range(0, len(stages), MAX_STAGES):
+                part = stages[chunk:chunk + MAX_STAGES]
+                fp = FusedPass()
+                fp.tile_bits = len(p.local)
+                for i, s in enumerate(p.local):
+                    fp.local_bits[i] = s
+                fp.num_stages = len(part)
+                first_op = len(op_structs)
+                needs_index = 0
+                tile_of = {s: i for i, s in enumerate(p.local)}
+                for si, (cube, ops) in enumerate(part):
+                    for k in range(CUBE_BITS):
+                        fp.stage_bits[si][k] = cube[k]
+                    fp.stage_ops[si] = len(op_structs) - first_op
+                    pos = {t: i for i, t in enumerate(cube)}
+                    for it in ops:
+                        o = FusedOp()
+                        if it.kind == "dense1":
+                            o.kind, o.a = FOP_DENSE1, pos[tile_of[it.bits[0]]]
+                        elif it.kind == "dense2":
+                            o.kind = FOP_DENSE2
+                            o.a, o.b = pos[tile_of[it.bits[0]]], pos[tile_of[it.bits[1]]]
+                        elif it.kind == "cnot":
+                            o.kind, o.a, o.b = FOP_CNOT, it.bits[0], pos[tile_of[it.bits[1]]]
+                            needs_index = 1
+                        elif it.kind == "cz":
+                            o.kind, o.a, o.b = FOP_CZ, it.bits[0], it.bits[1]
+                            needs_index = 1
+                        elif it.kind == "diag1":
+                            o.kind, o.a = FOP_DIAG1, it.bits[0]
+                            needs_index = 1
+                        elif it.kind == "phase":
+                            o.kind = FOP_PHASE
+                            needs_index = 1
+                        else:
+                            raise ValueError(it.kind)
+                        if it.data is not None:
+                            pending.append((len(op_structs), it))
+                        op_structs.append(o)
+                fp.stage_ops[len(part)] = len(op_structs) - first_op
+                fp.needs_index = needs_index
+                self.steps.append(("fused", fp, first_op, len(op_structs) - first_op))
+        for o in op_structs:
+            o.cache_slot = -1
+        # upload operand data and fill data pointers
+        for idx, it in pending:
+            o = op_structs[idx]
+            if it.kind == "phase":
+                t = torch.as_tensor(np.ascontiguousarray(it.data)).to(self.device, real)
+                o.data_bstride = 2 ** self.n if it.per_state else 0
+            else:
+                t = torch.as_tensor(np.ascontiguousarray(it.data)).to(self.device, dtype)
+                D = t.shape[-1] if it.kind != "diag1" else 2
+                o.data_bstride = (D * D if it.kind != "diag1" else 2) if it.per_state else 0
+            self._keep.append(t)
+            o.data = t.data_ptr()
+        # on-chip cache slots for data shared by every state, per pass
+        for st in self.steps:
+            if st[0] != "fused":
+                continue
+            _, fp, first, count = st
+            slot = 0
+            for o in op_structs[first:first + count]:
+                if o.kind in (FOP_DENSE1, FOP_DENSE2, FOP_DIAG1) and o.data_bstride == 0:
+                    size = {FOP_DENSE1: 4, FOP_DENSE2: 16, FOP_DIAG1: 2}[o.kind]
+                    if slot + size <= _lib.UA_FUSED_MATRIX_CACHE:
+                        o.cache_slot = slot
+                        slot += size
+        arr = (FusedOp * max(1, len(op_structs)))(*op_structs)
+        self._host_ops = arr
+        raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
+        self._ops_dev = raw.to(self.device)
+        base_ptr = self._ops_dev.data_ptr()
+        self._calls = []
+        for st in self.steps:
+            if st[0] == "fused":
+                _, fp, first, count = st
+                self._calls.append(("fused", fp, base_ptr + first * ctypes.sizeof(FusedOp),
+                                    ctypes.addressof(arr) + first * ctypes.sizeof(FusedOp)))
+            else:
+                _, qubits, m = st
+                self._calls.append(("wide", _lib.int_array(qubits), len(qubits), m))
+        self.num_passes = len(self._calls)
+        self.num_gates = len(circuit.gates)
+
+    def run(self, state: torch.Tensor) -> torch.Tensor:
+        """Apply the program to a contiguous (batch, 2**n) state in place."""
+        if state.device != self.device or state.dtype != self.dtype or \
+                state.shape[-1] != 2 ** self.n or not state.is_contiguous():
+            raise ValueError("state must be a contiguous (batch, 2**n) tensor of the "
+                             "program's dtype and device")
+        batch = state.numel() // (2 ** self.n)
+        stream = _lib.stream_handle(self.device)
+        for c in self._calls:
+            if c[0] == "fused":
+                rc = self.lib.ua_run_fused_pass(_lib.ptr(state), self.code, self.n, batch,
+                                                ctypes.byref(c[1]), ctypes.c_void_p(c[2]),
+                                                ctypes.c_void_p(c[3]), stream)
+            else:
+                rc = self.lib.ua_apply_operator(_lib.ptr(state), _lib.ptr(state),
+                                                _lib.ptr(c[3]), self.code, self.n, c[2],
+                                                c[1], batch, 2 ** self.n, 0, stream)
+            if rc:
+                _lib.check(rc, "fused program")
+        return state
+
+    def capture(self, state: torch.Tensor):
+        """Record run(state) into a CUDA/HIP graph; replay() re-runs it."""
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.run(state)
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self._graph = (g, state)
+        return g
+
+    def replay(self):
+        self._graph[0].replay()
+        return self._graph[1]

@@ -14,6 +14,29 @@ from .gates import haar_unitary
 _CNOT = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]], dtype=np.complex128)
 
 
+class SeededAngles:
+    """Full-length phase angles drawn on first use from their own seed:
+    uniform [0, 2pi) float32 values, 2**n of them.  Keeps circuit construction
+    cheap when the angles are never looked at (planning, tests)."""
+
+    def __init__(self, n: int, seed: int):
+        self.n, self.seed = n, int(seed)
+        self._values = None
+
+    def values(self) -> np.ndarray:
+        if self._values is None:
+            rng = np.random.default_rng(self.seed)
+            self._values = rng.random(2 ** self.n, dtype=np.float32) * np.float32(2 * np.pi)
+        return self._values
+
+    def __array__(self, dtype=None, copy=None):
+        v = self.values()
+        return v if dtype is None else v.astype(dtype)
+
+    def __len__(self):
+        return 2 ** self.n
+
+
 def single_qubit_phase_angles(n: int, qubit: int, phi0: float, phi1: float) -> np.ndarray:
     """Full-length apply_phase angles of a diagonal gate on one qubit:
     angle[x] = phi0 if that qubit's bit of x is 0 else phi1."""
@@ -64,7 +87,6 @@ def random_circuit_24q(seed: int = 2, n: int = 24, n_1q: int = 120, n_2q_each: i
     for _ in range(n_2q_each):
         gates.append(Gate("dense", pair(), matrix=haar_unitary(4, rng)))
     for _ in range(n_phase):
-        ang = rng.random(2 ** n, dtype=np.float32) * np.float32(2 * np.pi)
-        gates.append(Gate("phase", (), angles=ang))
+        gates.append(Gate("phase", (), angles=SeededAngles(n, rng.integers(2 ** 62))))
     order = rng.permutation(len(gates))
     return Circuit(n, [gates[i] for i in order])

@@ -1,0 +1,97 @@
+"""CPU checks of the fused-pass compiler (unitair_amd.fusion): the plan keeps
+every gate, respects the local-bit and cube limits, and the 1-qubit merging
+step leaves the circuit's result unchanged (checked with the oracle)."""
+import numpy as np
+import pytest
+
+from unitair_amd import fusion
+from unitair_amd.circuits import Circuit, Gate
+from unitair_amd.gates import haar_unitary
+from unitair_amd.workloads import SeededAngles, layered_circuit_12q, random_circuit_24q
+from oracle import circuits as oc
+
+
+def random_mixed_circuit(n, ngates, seed):
+    rng = np.random.default_rng(seed)
+    gates = []
+    for _ in range(ngates):
+        r = rng.random()
+        if r < 0.45:
+            gates.append(Gate("dense", (int(rng.integers(n)),), matrix=haar_unitary(2, rng)))
+        elif r < 0.6:
+            a, b = rng.choice(n, 2, replace=False)
+            gates.append(Gate("dense", (int(a), int(b)), matrix=haar_unitary(4, rng)))
+        elif r < 0.7:
+            a, b = rng.choice(n, 2, replace=False)
+            gates.append(Gate("cnot", (int(a), int(b))))
+        elif r < 0.8:
+            a, b = rng.choice(n, 2, replace=False)
+            gates.append(Gate("cz", (int(a), int(b))))
+        elif r < 0.9:
+            gates.append(Gate("phase", (), angles=SeededAngles(n, rng.integers(2 ** 62))))
+        else:
+            q = rng.choice(n, 3, replace=False)
+            gates.append(Gate("dense", tuple(int(x) for x in q), matrix=haar_unitary(8, rng)))
+    return Circuit(n, gates)
+
+
+def items_to_gates(items, n):
+    gates = []
+    for it in items:
+        qubits = tuple(n - 1 - b for b in it.bits)
+        if it.kind in ("dense1", "dense2", "wide"):
+            gates.append(Gate("dense", qubits, matrix=it.data))
+        elif it.kind == "phase":
+            gates.append(Gate("phase", (), angles=it.data))
+        else:
+            gates.append(Gate(it.kind, qubits))
+    return gates
+
+
+@pytest.mark.parametrize("n,seed", [(5, 0), (8, 1), (12, 2), (14, 3)])
+def test_merge_preserves_result(n, seed):
+    circ = random_mixed_circuit(n, 60, seed)
+    items = fusion._items_from_circuit(circ)
+    merged = fusion._merge_1q(items)
+    assert len(merged) <= len(items)
+    rng = np.random.default_rng(seed + 100)
+    psi = rng.standard_normal(2 ** n) + 1j * rng.standard_normal(2 ** n)
+    a = oc.run_closed_form(circ.gates, psi)
+    b = oc.run_closed_form(items_to_gates(merged, n), psi)
+    assert np.allclose(a, b, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,seed", [(4, 0), (7, 1), (12, 2), (16, 3)])
+def test_plan_respects_limits(n, seed):
+    circ = random_mixed_circuit(n, 120, seed)
+    items = fusion._merge_1q(fusion._items_from_circuit(circ))
+    plan = fusion.plan_passes(items, n)
+    m = min(n, fusion.TILE_BITS)
+    flat = []
+    for p in plan:
+        if isinstance(p, fusion._Item):
+            assert p.kind == "wide"
+            flat.append(p)
+            continue
+        assert len(p.local) == m and p.local == sorted(p.local)
+        assert p.local[:4] == [0, 1, 2, 3]
+        for it in p.items:
+            assert fusion._needs(it) <= set(p.local)
+        stages = fusion._stages(p)
+        seen = []
+        for cube, ops in stages:
+            assert len(cube) == 4 and cube == sorted(cube) and all(0 <= t < m for t in cube)
+            tile_of = {s: i for i, s in enumerate(p.local)}
+            for it in ops:
+                assert {tile_of[b] for b in fusion._needs(it)} <= set(cube)
+            seen += ops
+        assert seen == p.items
+        flat += p.items
+    assert [id(x) for x in flat] == [id(x) for x in items]      # order and count kept
+
+
+def test_24q_circuit_needs_few_passes():
+    circ = random_circuit_24q(seed=2)
+    items = fusion._merge_1q(fusion._items_from_circuit(circ))
+    plan = fusion.plan_passes(items, 24)
+    assert len(plan) < 40

@@ -1,0 +1,68 @@
+"""Fused passes (ua_run_fused_pass via unitair_amd.fusion) against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from unitair_amd.fusion import FusedProgram
+from unitair_amd.initializations import seeded_state_np
+from oracle import circuits as oc
+from oracle import unitair_np as U
+from test_fusion_plan import random_mixed_circuit
+
+pytestmark = pytest.mark.gpu
+TOL = {torch.complex64: 1e-5, torch.complex128: 1e-12}
+
+
+def rel_err(out, ref):
+    return float(np.max(np.abs(out - ref)) / max(1e-300, np.max(np.abs(ref))))
+
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("n,batch", [(4, 3), (5, 1), (8, 2), (11, 1), (12, 3), (13, 1), (16, 2), (20, 1)])
+def test_fused_random_circuits(device, dt, n, batch):
+    circ = random_mixed_circuit(n, 80, seed=n * 7 + batch)
+    prog = FusedProgram(circ, dtype=dt, device=device)
+    psi = seeded_state_np(n, (batch,), seed=n)
+    ref = np.stack([oc.run_closed_form(circ.gates, p) for p in psi])
+    x = torch.from_numpy(psi).to(device, dt)
+    out = prog.run(x).cpu().numpy()
+    assert rel_err(out, ref) < TOL[dt] * 10, (prog.num_passes, n)
+    assert np.max(U.fidelity_error(ref, out)) < TOL[dt]
+
+
+def test_fused_layered_12q_batch8_c128(device):
+    from unitair_amd.workloads import layered_circuit_12q
+    circ = layered_circuit_12q(seed=1)
+    psi = seeded_state_np(12, (8,), seed=11)
+    ref = np.stack([oc.run_closed_form(circ.gates, p) for p in psi])
+    prog = FusedProgram(circ, dtype=torch.complex128, device=device)
+    out = prog.run(torch.from_numpy(psi).to(device)).cpu().numpy()
+    assert rel_err(out, ref) < 1e-12
+
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
+def test_fused_random_circuit_24q(device, dt):
+    from test_gpu_parity import _circuit_24q_and_oracle
+    circ, psi, ref = _circuit_24q_and_oracle()
+    prog = FusedProgram(circ, dtype=dt, device=device)
+    out = prog.run(torch.from_numpy(psi).to(device, dt)).cpu().numpy()
+    fid = float(U.fidelity_error(ref, out))
+    mae = U.max_abs_error(ref, out)
+    print(f"fused 24q {dt}: {prog.num_passes} passes, fidelity error {fid:.3e}, max-abs {mae:.3e}")
+    assert fid < TOL[dt] and mae < TOL[dt]
+
+
+def test_fused_graph_replay_matches_eager(device):
+    n = 14
+    circ = random_mixed_circuit(n, 60, seed=5)
+    prog = FusedProgram(circ, dtype=torch.complex64, device=device)
+    psi = torch.from_numpy(seeded_state_np(n, (2,), seed=3)).to(device, torch.complex64)
+    eager = prog.run(psi.clone())
+    state = psi.clone()
+    prog.capture(state)
+    for _ in range(3):
+        state.copy_(psi)                 # eager work between replays
+        torch.cuda.synchronize()
+        prog.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(state, eager)
