@@ -257,6 +257,48 @@ int ua_run_fused_pass(void* state, int dtype, int num_qubits, int64_t batch,
                       const ua_fused_pass* pass, const ua_fused_op* ops,
                       const ua_fused_op* host_ops, ua_stream_t stream);
 
+/*
+ * Differentiation.  unitair relies on torch autograd through its op sequence
+ * (operations.py:41-42,322) and keeps one saved state per gate.
+ *
+ * ua_operator_gradient -- backward of apply_operator w.r.t. the operator:
+ *   grad_op[o][r][c] = sum over batch entries b using operator o and over the
+ *   2^(n-k) groups:  grad_out[b][.. r ..] * conj(state[b][.. c ..])
+ * op_count is 1 (one operator for the whole batch) or batch.  grad_out is
+ * contiguous (batch, 2^n); state rows start at state + b*state_batch_stride.
+ * k <= UA_MAX_GRAD_QUBITS.  The gradient w.r.t. the state is
+ * ua_apply_operator with the conjugate-transposed operator.
+ */
+#define UA_MAX_GRAD_QUBITS 16
+int ua_operator_gradient(void* grad_op, const void* grad_out, const void* state, int dtype,
+                         int num_qubits, int op_num_qubits, const int* qubits,
+                         int64_t batch, int64_t op_count, int64_t state_batch_stride,
+                         ua_stream_t stream);
+
+/*
+ * Adjoint differentiation of Pauli-rotation circuits for H = sum_i Z_i (the
+ * loss unitair computes as diag_expectation_value(n - 2*popcount(x), psi),
+ * innerprod.py:49-59).  No reference counterpart beyond that loss.
+ *
+ * ua_expect_sum_z   out[b] = sum_x |psi_b(x)|^2 (n - 2 popcount x)   (float64)
+ * ua_apply_sum_z    out = H psi (the co-state at the end of the circuit)
+ * ua_pauli_gradients out[b*n + q] = Im <lambda_b| P_q |psi_b> for every qubit
+ *                   q, P = Y or Z: the derivative of the loss w.r.t. the angle
+ *                   t of a rotation exp(-i t/2 P_q) sitting where (lambda, psi)
+ *                   were taken.  Y needs num_qubits <= 16.
+ * Workspace: ua_pauli_gradients_workspace(num_qubits, batch) bytes (both
+ * ua_expect_sum_z and ua_pauli_gradients).  complex64/complex128 states.
+ */
+enum { UA_PAULI_Y = 1, UA_PAULI_Z = 2 };
+int64_t ua_pauli_gradients_workspace(int num_qubits, int64_t batch);
+int ua_expect_sum_z(double* out, const void* psi, int dtype, int num_qubits, int64_t batch,
+                    void* workspace, ua_stream_t stream);
+int ua_apply_sum_z(void* out, const void* psi, int dtype, int num_qubits, int64_t batch,
+                   ua_stream_t stream);
+int ua_pauli_gradients(double* out, const void* lambda, const void* psi, int dtype,
+                       int num_qubits, int64_t batch, int pauli, void* workspace,
+                       ua_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

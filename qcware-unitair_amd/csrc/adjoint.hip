@@ -1,0 +1,461 @@
+// Kernels for adjoint differentiation of Pauli-rotation circuits with the
+// observable H = sum_i Z_i.
+//
+// unitair has no gradient code: its circuits are differentiated by torch
+// autograd through apply_operator / apply_phase (unitair/simulation/
+// operations.py:41-42,322), the gate builders (unitair/gates/gates.py:35-91)
+// and the loss diag_expectation_value (unitair/states/innerprod.py:49-59),
+// which keeps one saved state per gate.  The adjoint method keeps two states
+// (psi and the co-state lambda = H psi pulled back through the gates) and
+// needs, per rotation exp(-i t/2 P_q):
+//     dL/dt = Im <lambda| P_q |psi>
+// evaluated at the point right after that gate.  Same-kind rotations on
+// different qubits commute, so one (lambda, psi) point serves every qubit of a
+// layer: ua_pauli_gradients returns all n values of a layer in one call.
+#include "ua_common.h"
+
+namespace ua {
+namespace {
+
+constexpr int kLowBits = 12;
+constexpr int kLowTile = 1 << kLowBits;
+constexpr int kMaxN = 40;
+constexpr int kZChunkConst = 4096;
+
+template <typename T> struct D2;
+template <> struct D2<float2> {
+  __device__ static double2 get(float2 v) { return make_double2(v.x, v.y); }
+};
+template <> struct D2<double2> {
+  __device__ static double2 get(double2 v) { return v; }
+};
+
+// Re(conj(a) * b)
+__device__ __forceinline__ double re_cj(double2 a, double2 b) { return a.x * b.x + a.y * b.y; }
+// Im(conj(a) * b)
+__device__ __forceinline__ double im_cj(double2 a, double2 b) { return a.x * b.y - a.y * b.x; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Sums v over the workgroup (256 threads); result in thread 0.
+__device__ double block_sum(double v, double* scratch) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kBlock / 64; ++w) r += scratch[w];
+  return r;
+}
+
+// Y gradients for index bits 0..min(n,12)-1.  One workgroup per (state,
+// tile of 2^m contiguous amplitudes); lambda and psi tiles sit in LDS.
+// partial[(b * tiles + tile) * n + bit] = sum over the tile's pairs.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void grad_y_low_kernel(
+    double* __restrict__ partial, const T* __restrict__ lam, const T* __restrict__ psi,
+    int n, int m, int64_t tiles_per_state) {
+  __shared__ T sl[kLowTile];
+  __shared__ T sp[kLowTile];
+  __shared__ double scratch[kBlock / 64];
+  const int64_t tile = blockIdx.x;
+  const int64_t b = tile / tiles_per_state;
+  const int64_t t = tile % tiles_per_state;
+  const int64_t N = int64_t(1) << n;
+  const int size = 1 << m;
+  const T* L = lam + b * N + t * size;
+  const T* P = psi + b * N + t * size;
+  for (int i = threadIdx.x; i < size; i += kBlock) {
+    sl[i] = L[i];
+    sp[i] = P[i];
+  }
+  __syncthreads();
+  for (int bit = 0; bit < m; ++bit) {
+    double acc = 0.0;
+    const int half = size >> 1;
+    for (int pidx = threadIdx.x; pidx < half; pidx += kBlock) {
+      const int x0 = ((pidx >> bit) << (bit + 1)) | (pidx & ((1 << bit) - 1));
+      const int x1 = x0 | (1 << bit);
+      const double2 l0 = D2<T>::get(sl[x0]), l1 = D2<T>::get(sl[x1]);
+      const double2 p0 = D2<T>::get(sp[x0]), p1 = D2<T>::get(sp[x1]);
+      acc += re_cj(l1, p0) - re_cj(l0, p1);
+    }
+    const double s = block_sum(acc, scratch);
+    if (threadIdx.x == 0) partial[(b * tiles_per_state + t) * n + bit] = s;
+  }
+}
+
+// Y gradients for index bits >= 12 (n <= 16): each thread owns one low index
+// and the 2^(n-12) amplitudes that share it, read with a 2^12 stride.
+// partial[(b * 16 + blockIdx.y) * n + bit] for bits 12..n-1.
+template <typename T, int H>
+__global__ __launch_bounds__(kBlock) void grad_y_high_kernel(
+    double* __restrict__ partial, const T* __restrict__ lam, const T* __restrict__ psi, int n) {
+  __shared__ double scratch[kBlock / 64];
+  constexpr int HN = 1 << H;
+  const int64_t b = blockIdx.x;
+  const int low = blockIdx.y * kBlock + threadIdx.x;  // < 4096
+  const int64_t N = int64_t(1) << n;
+  double2 l[HN], p[HN];
+#pragma unroll
+  for (int h = 0; h < HN; ++h) {
+    l[h] = D2<T>::get(lam[b * N + (int64_t(h) << kLowBits) + low]);
+    p[h] = D2<T>::get(psi[b * N + (int64_t(h) << kLowBits) + low]);
+  }
+#pragma unroll
+  for (int hb = 0; hb < H; ++hb) {
+    double acc = 0.0;
+#pragma unroll
+    for (int h = 0; h < HN; ++h) {
+      if (h & (1 << hb)) continue;
+      const int h1 = h | (1 << hb);
+      acc += re_cj(l[h1], p[h]) - re_cj(l[h], p[h1]);
+    }
+    const double s = block_sum(acc, scratch);
+    if (threadIdx.x == 0) partial[(b * gridDim.y + blockIdx.y) * n + kLowBits + hb] = s;
+  }
+}
+
+// Z gradients (and, with want_loss, the loss sum_x |psi_x|^2 (n-2 popcount x)).
+// A workgroup covers a chunk of 4096 amplitudes of one state (or the whole
+// state when it is smaller); thread t visits i = chunk_base + t + 256*k.
+// Index bits 0..7 of i are fixed per thread, bits 8..11 follow k and bits
+// >= 12 are fixed per chunk, so each thread keeps 16 partial sums (one per k)
+// and the per-qubit signs are applied afterwards.
+// partial[(b * chunks + c) * (n + 1) + q]; slot n holds the loss part.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void grad_z_kernel(
+    double* __restrict__ partial, const T* __restrict__ lam, const T* __restrict__ psi, int n,
+    int64_t chunks, int want_loss) {
+  __shared__ double scratch[kBlock / 64];
+  constexpr int K = kZChunkConst / kBlock;  // 16
+  const int64_t b = blockIdx.x / chunks;
+  const int64_t c = blockIdx.x % chunks;
+  const int64_t N = int64_t(1) << n;
+  const int64_t base = c * kZChunkConst;
+  double vk[K];
+  double loss = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    vk[k] = 0.0;
+    const int64_t i = base + threadIdx.x + int64_t(k) * kBlock;
+    if (i < N) {
+      const double2 pv = D2<T>::get(psi[b * N + i]);
+      if (want_loss) {
+        const double w = pv.x * pv.x + pv.y * pv.y;
+        loss += w * double(n - 2 * int(__popcll(uint64_t(i))));
+      }
+      if (lam) vk[k] = im_cj(D2<T>::get(lam[b * N + i]), pv);
+    }
+  }
+  double total = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) total += vk[k];
+  if (want_loss) {
+    const double sl = block_sum(loss, scratch);
+    if (threadIdx.x == 0) partial[(b * chunks + c) * (n + 1) + n] = sl;
+  }
+  if (!lam) return;
+  for (int bit = 0; bit < n; ++bit) {
+    double v;
+    if (bit < 8) {
+      v = ((threadIdx.x >> bit) & 1) ? -total : total;
+    } else if (bit < 12) {
+      v = 0.0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) v += ((k >> (bit - 8)) & 1) ? -vk[k] : vk[k];
+    } else {
+      v = ((base >> bit) & 1) ? -total : total;
+    }
+    const double sum = block_sum(v, scratch);
+    if (threadIdx.x == 0) partial[(b * chunks + c) * (n + 1) + (n - 1 - bit)] = sum;
+  }
+}
+
+// out[b * n_out + q] = sum over `parts` rows of partial[(b*parts + r)*stride + q0 + q]
+__global__ __launch_bounds__(kBlock) void sum_parts_kernel(
+    double* __restrict__ out, const double* __restrict__ partial, int64_t batch, int64_t parts,
+    int stride, int q0, int n_out, int out_stride, int out_q0, int accumulate) {
+  const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= batch * n_out) return;
+  const int64_t b = idx / n_out;
+  const int q = int(idx % n_out);
+  double s = 0.0;
+  for (int64_t r = 0; r < parts; ++r) s += partial[(b * parts + r) * stride + q0 + q];
+  double* o = out + b * out_stride + out_q0 + q;
+  *o = accumulate ? *o + s : s;
+}
+
+// lambda_x = (n - 2 popcount(x)) psi_x
+template <typename T>
+__global__ __launch_bounds__(kBlock) void sum_z_apply_kernel(T* __restrict__ out,
+                                                             const T* __restrict__ psi, int n,
+                                                             int64_t total) {
+  const int64_t stride = int64_t(gridDim.x) * kBlock;
+  const uint64_t mask = (uint64_t(1) << n) - 1;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < total; i += stride) {
+    const auto w = decltype(psi[0].x)(n - 2 * int(__popcll(uint64_t(i) & mask)));
+    T v = psi[i];
+    v.x *= w;
+    v.y *= w;
+    out[i] = v;
+  }
+}
+
+int64_t y_low_parts(int n) { return n > kLowBits ? (int64_t(1) << (n - kLowBits)) : 1; }
+constexpr int64_t kHighParts = kLowTile / kBlock;  // 16
+int64_t z_parts(int n) {
+  return ((int64_t(1) << n) + kZChunkConst - 1) / kZChunkConst;
+}
+
+}  // namespace
+}  // namespace ua
+
+using namespace ua;
+
+extern "C" int64_t ua_pauli_gradients_workspace(int num_qubits, int64_t batch) {
+  const int n = num_qubits;
+  if (n < 1 || n > kMaxN || batch < 0) return -1;
+  int64_t a = y_low_parts(n) * n;
+  const int64_t h = kHighParts * n;
+  const int64_t z = z_parts(n) * (n + 1);
+  if (h > a) a = h;
+  if (z > a) a = z;
+  return (batch < 1 ? 1 : batch) * a * int64_t(sizeof(double));
+}
+
+extern "C" int ua_pauli_gradients(double* out, const void* lambda, const void* psi, int dtype,
+                                  int num_qubits, int64_t batch, int pauli, void* workspace,
+                                  ua_stream_t stream) {
+  const int n = num_qubits;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  UA_CHECK_ARG(dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128, "complex states only");
+  UA_CHECK_ARG(n >= 1 && n <= kMaxN, "num_qubits=%d out of range", n);
+  UA_CHECK_ARG(pauli == UA_PAULI_Y || pauli == UA_PAULI_Z, "pauli must be Y or Z");
+  UA_CHECK_ARG(batch >= 0, "negative batch");
+  if (batch == 0) return UA_OK;
+  UA_CHECK_ARG(out && lambda && psi && workspace, "NULL pointer");
+  double* part = static_cast<double*>(workspace);
+  const bool c64 = dtype == UA_COMPLEX64;
+  if (pauli == UA_PAULI_Z) {
+    const int64_t chunks = z_parts(n);
+    UA_CHECK_ARG(batch * chunks <= 0x7fffffffLL, "grid too large");
+    const unsigned g = unsigned(batch * chunks);
+    if (c64)
+      grad_z_kernel<float2><<<g, kBlock, 0, s>>>(part, (const float2*)lambda, (const float2*)psi,
+                                                 n, chunks, 0);
+    else
+      grad_z_kernel<double2><<<g, kBlock, 0, s>>>(part, (const double2*)lambda,
+                                                  (const double2*)psi, n, chunks, 0);
+    int rc = check_launch("grad_z_kernel");
+    if (rc) return rc;
+    // out[b*n + q] = sum over chunks (partials are stored by qubit)
+    sum_parts_kernel<<<grid_for(batch * n), kBlock, 0, s>>>(out, part, batch, chunks, n + 1, 0, n,
+                                                            n, 0, 0);
+    return check_launch("sum_parts_kernel");
+  }
+  // Y: index bit i belongs to qubit n-1-i; partials are kept per index bit
+  // and mapped to qubits when summed.
+  const int m = n < kLowBits ? n : kLowBits;
+  const int64_t tiles = y_low_parts(n);
+  UA_CHECK_ARG(batch * tiles <= 0x7fffffffLL, "grid too large");
+  if (c64)
+    grad_y_low_kernel<float2><<<unsigned(batch * tiles), kBlock, 0, s>>>(
+        part, (const float2*)lambda, (const float2*)psi, n, m, tiles);
+  else
+    grad_y_low_kernel<double2><<<unsigned(batch * tiles), kBlock, 0, s>>>(
+        part, (const double2*)lambda, (const double2*)psi, n, m, tiles);
+  int rc = check_launch("grad_y_low_kernel");
+  if (rc) return rc;
+  // bits 0..m-1 -> qubits n-1 .. n-m   (sum_parts writes q-th bit at out index)
+  for (int bit = 0; bit < m; ++bit) {
+    sum_parts_kernel<<<grid_for(batch), kBlock, 0, s>>>(out, part, batch, tiles, n, bit, 1, n,
+                                                        n - 1 - bit, 0);
+    rc = check_launch("sum_parts_kernel");
+    if (rc) return rc;
+  }
+  if (n > kLowBits) {
+    const int H = n - kLowBits;
+    UA_CHECK_ARG(H <= 4, "Y gradients support up to 16 qubits");
+    UA_CHECK_ARG(batch <= 0x7fffffffLL, "grid too large");
+    const dim3 grid = dim3(static_cast<unsigned>(batch), static_cast<unsigned>(kHighParts), 1u);
+#define UA_HIGH(HH)                                                                       \
+  if (H == HH) {                                                                         \
+    if (c64)                                                                             \
+      grad_y_high_kernel<float2, HH><<<grid, kBlock, 0, s>>>(part, (const float2*)lambda, \
+                                                             (const float2*)psi, n);     \
+    else                                                                                 \
+      grad_y_high_kernel<double2, HH><<<grid, kBlock, 0, s>>>(                           \
+          part, (const double2*)lambda, (const double2*)psi, n);                         \
+  }
+    UA_HIGH(1) UA_HIGH(2) UA_HIGH(3) UA_HIGH(4)
+#undef UA_HIGH
+    rc = check_launch("grad_y_high_kernel");
+    if (rc) return rc;
+    for (int bit = kLowBits; bit < n; ++bit) {
+      sum_parts_kernel<<<grid_for(batch), kBlock, 0, s>>>(out, part, batch, kHighParts, n, bit, 1,
+                                                          n, n - 1 - bit, 0);
+      rc = check_launch("sum_parts_kernel");
+      if (rc) return rc;
+    }
+  }
+  return UA_OK;
+}
+
+extern "C" int ua_apply_sum_z(void* out, const void* psi, int dtype, int num_qubits,
+                              int64_t batch, ua_stream_t stream) {
+  const int n = num_qubits;
+  UA_CHECK_ARG(dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128, "complex states only");
+  UA_CHECK_ARG(n >= 1 && n <= 62 && batch >= 0, "bad sizes");
+  if (batch == 0) return UA_OK;
+  UA_CHECK_ARG(out && psi, "NULL pointer");
+  const int64_t total = batch << n;
+  int64_t blocks = (total + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == UA_COMPLEX64)
+    sum_z_apply_kernel<float2><<<unsigned(blocks), kBlock, 0, s>>>((float2*)out,
+                                                                   (const float2*)psi, n, total);
+  else
+    sum_z_apply_kernel<double2><<<unsigned(blocks), kBlock, 0, s>>>((double2*)out,
+                                                                    (const double2*)psi, n, total);
+  return check_launch("sum_z_apply_kernel");
+}
+
+extern "C" int ua_expect_sum_z(double* out, const void* psi, int dtype, int num_qubits,
+                               int64_t batch, void* workspace, ua_stream_t stream) {
+  const int n = num_qubits;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  UA_CHECK_ARG(dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128, "complex states only");
+  UA_CHECK_ARG(n >= 1 && n <= kMaxN && batch >= 0, "bad sizes");
+  if (batch == 0) return UA_OK;
+  UA_CHECK_ARG(out && psi && workspace, "NULL pointer");
+  double* part = static_cast<double*>(workspace);
+  const int64_t chunks = z_parts(n);
+  UA_CHECK_ARG(batch * chunks <= 0x7fffffffLL, "grid too large");
+  const unsigned g = unsigned(batch * chunks);
+  if (dtype == UA_COMPLEX64)
+    grad_z_kernel<float2><<<g, kBlock, 0, s>>>(part, nullptr, (const float2*)psi, n, chunks, 1);
+  else
+    grad_z_kernel<double2><<<g, kBlock, 0, s>>>(part, nullptr, (const double2*)psi, n, chunks, 1);
+  int rc = check_launch("grad_z_kernel");
+  if (rc) return rc;
+  sum_parts_kernel<<<grid_for(batch), kBlock, 0, s>>>(out, part, batch, chunks, n + 1, n, 1, 1, 0,
+                                                      0);
+  return check_launch("sum_parts_kernel");
+}
+
+// ---------------------------------------------------------------------------
+// Gradient of apply_operator with respect to the operator (backward of
+// unitair/simulation/operations.py:322, out = U @ in on the target bits):
+//   grad_U[o][r][c] = sum over batch entries b using operator o, and over
+//                     groups g:  grad_out[b][g|off(r)] * conj(in[b][g|off(c)])
+// One workgroup per (o, r, c); float64 accumulation.
+namespace ua {
+namespace {
+
+struct OpGradParams {
+  int n, k;
+  int64_t batch, op_count;   // op_count: distinct operators (1 or batch)
+  int64_t in_bstride;        // 0 when one input state serves the whole batch
+  int tbit[UA_MAX_GRAD_QUBITS];  // state bit of gate-index bit k-1-m (qubits order)
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void op_grad_kernel(
+    T* __restrict__ grad_op, const T* __restrict__ gout, const T* __restrict__ in,
+    OpGradParams p) {
+  __shared__ double scratch[kBlock / 64];
+  const int64_t D = int64_t(1) << p.k;
+  const int64_t o = blockIdx.x / (D * D);
+  const int64_t r = (blockIdx.x / D) % D;
+  const int64_t c = blockIdx.x % D;
+  uint64_t offr = 0, offc = 0, mask = 0;
+  for (int m = 0; m < p.k; ++m) {
+    const uint64_t bit = uint64_t(1) << p.tbit[m];
+    mask |= bit;
+    if ((r >> (p.k - 1 - m)) & 1) offr |= bit;
+    if ((c >> (p.k - 1 - m)) & 1) offc |= bit;
+  }
+  const int64_t N = int64_t(1) << p.n;
+  const int64_t groups = N >> p.k;
+  const int64_t per_op = p.batch / p.op_count;   // batch entries sharing operator o
+  double sr = 0.0, si = 0.0;
+  for (int64_t t = threadIdx.x; t < per_op * groups; t += kBlock) {
+    const int64_t b = (p.op_count == 1) ? t / groups : o;
+    const int64_t gi = t % groups;
+    // insert zeros at the target bits (ascending order of bit position)
+    uint64_t base = uint64_t(gi);
+    for (int bitpos = 0; bitpos < p.n; ++bitpos)
+      if ((mask >> bitpos) & 1) {
+        const uint64_t low = base & ((uint64_t(1) << bitpos) - 1);
+        base = ((base >> bitpos) << (bitpos + 1)) | low;
+      }
+    if constexpr (Num<T>::complex) {
+      const T g = gout[b * N + (base | offr)];
+      const T x = in[b * p.in_bstride + (base | offc)];
+      sr += double(g.x) * x.x + double(g.y) * x.y;    // g * conj(x)
+      si += double(g.y) * x.x - double(g.x) * x.y;
+    } else {
+      sr += double(gout[b * N + (base | offr)]) * in[b * p.in_bstride + (base | offc)];
+    }
+  }
+  const double tr = block_sum(sr, scratch);
+  const double ti = block_sum(si, scratch);
+  if (threadIdx.x == 0) {
+    if constexpr (Num<T>::complex) {
+      T v;
+      v.x = tr;
+      v.y = ti;
+      grad_op[blockIdx.x] = v;
+    } else {
+      grad_op[blockIdx.x] = T(tr);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace ua
+
+extern "C" int ua_operator_gradient(void* grad_op, const void* grad_out, const void* state,
+                                    int dtype, int num_qubits, int op_num_qubits,
+                                    const int* qubits, int64_t batch, int64_t op_count,
+                                    int64_t state_batch_stride, ua_stream_t stream) {
+  using namespace ua;
+  const int n = num_qubits, k = op_num_qubits;
+  UA_CHECK_ARG(elem_size(dtype) != 0, "bad dtype");
+  UA_CHECK_ARG(n >= 1 && n <= 62 && k >= 0 && k <= n && k <= UA_MAX_GRAD_QUBITS,
+               "bad qubit counts (n=%d, k=%d)", n, k);
+  UA_CHECK_ARG(batch >= 0 && (op_count == 1 || op_count == batch), "op_count must be 1 or batch");
+  if (batch == 0) return UA_OK;
+  UA_CHECK_ARG(grad_op && grad_out && state, "NULL pointer");
+  OpGradParams p{};
+  p.n = n;
+  p.k = k;
+  p.batch = batch;
+  p.op_count = op_count;
+  p.in_bstride = state_batch_stride;
+  uint64_t seen = 0;
+  for (int m = 0; m < k; ++m) {
+    UA_CHECK_ARG(qubits[m] >= 0 && qubits[m] < n, "qubit out of range");
+    p.tbit[m] = n - 1 - qubits[m];
+    UA_CHECK_ARG(!((seen >> p.tbit[m]) & 1), "duplicate qubit");
+    seen |= uint64_t(1) << p.tbit[m];
+  }
+  const int64_t blocks = op_count * (int64_t(1) << (2 * k));
+  UA_CHECK_ARG(blocks <= 0x7fffffffLL, "grid too large");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case UA_FLOAT32: op_grad_kernel<float><<<unsigned(blocks), kBlock, 0, s>>>((float*)grad_op, (const float*)grad_out, (const float*)state, p); break;
+    case UA_FLOAT64: op_grad_kernel<double><<<unsigned(blocks), kBlock, 0, s>>>((double*)grad_op, (const double*)grad_out, (const double*)state, p); break;
+    case UA_COMPLEX64: op_grad_kernel<float2><<<unsigned(blocks), kBlock, 0, s>>>((float2*)grad_op, (const float2*)grad_out, (const float2*)state, p); break;
+    case UA_COMPLEX128: op_grad_kernel<double2><<<unsigned(blocks), kBlock, 0, s>>>((double2*)grad_op, (const double2*)grad_out, (const double2*)state, p); break;
+  }
+  return check_launch("op_grad_kernel");
+}

@@ -165,14 +165,32 @@ def _stages(p: _Pass):
 
 
 class FusedProgram:
-    """A circuit compiled into fused passes for one (n, dtype, device)."""
+    """A circuit compiled into fused passes for one (n, dtype, device).
+
+    Built from a Circuit, or with from_items() from gate items whose data may
+    be per state (a device tensor with a leading batch dimension)."""
 
     def __init__(self, circuit: Circuit, dtype: torch.dtype = torch.complex64,
                  device="cuda", tile_bits: int = TILE_BITS, merge_1q: bool = True):
+        items = _items_from_circuit(circuit)
+        if merge_1q:
+            items = _merge_1q(items)
+        self._build(items, circuit.num_qubits, dtype, device, tile_bits)
+        self.num_gates = len(circuit.gates)
+
+    @classmethod
+    def from_items(cls, items: List[_Item], n: int, dtype: torch.dtype = torch.complex64,
+                   device="cuda", tile_bits: int = TILE_BITS) -> "FusedProgram":
+        prog = cls.__new__(cls)
+        prog._build(list(items), n, dtype, device, tile_bits)
+        prog.num_gates = len(items)
+        return prog
+
+    def _build(self, items, n, dtype, device, tile_bits):
         if dtype not in (torch.complex64, torch.complex128):
             raise ValueError("fused programs run complex64/complex128 states")
         self.lib = _lib.load()
-        self.n = circuit.num_qubits
+        self.n = n
         if self.n < CUBE_BITS:
             raise ValueError("fused programs need at least 4 qubits")
         self.dtype = dtype
@@ -181,9 +199,6 @@ class FusedProgram:
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         real = torch.float32 if dtype == torch.complex64 else torch.float64
-        items = _items_from_circuit(circuit)
-        if merge_1q:
-            items = _merge_1q(items)
         plan = plan_passes(items, self.n, tile_bits)
         self._keep = []
         self.steps = []          # ("fused", FusedPass, ops_ptr, nops) | ("wide", qubits, mat)
@@ -244,13 +259,16 @@ class FusedProgram:
         # upload operand data and fill data pointers
         for idx, it in pending:
             o = op_structs[idx]
-            if it.kind == "phase":
-                t = torch.as_tensor(np.ascontiguousarray(it.data)).to(self.device, real)
-                o.data_bstride = 2 ** self.n if it.per_state else 0
+            want = real if it.kind == "phase" else dtype
+            if isinstance(it.data, torch.Tensor):
+                t = it.data
+                if t.device != self.device or t.dtype != want or not t.is_contiguous():
+                    raise ValueError("device gate data must be contiguous, of the program's "
+                                     "dtype and device")
             else:
-                t = torch.as_tensor(np.ascontiguousarray(it.data)).to(self.device, dtype)
-                D = t.shape[-1] if it.kind != "diag1" else 2
-                o.data_bstride = (D * D if it.kind != "diag1" else 2) if it.per_state else 0
+                t = torch.as_tensor(np.ascontiguousarray(it.data)).to(self.device, want)
+            per = {"phase": 2 ** self.n, "diag1": 2, "dense1": 4, "dense2": 16}[it.kind]
+            o.data_bstride = per if it.per_state else 0
             self._keep.append(t)
             o.data = t.data_ptr()
         # on-chip cache slots for data shared by every state, per pass
@@ -280,7 +298,6 @@ class FusedProgram:
                 _, qubits, m = st
                 self._calls.append(("wide", _lib.int_array(qubits), len(qubits), m))
         self.num_passes = len(self._calls)
-        self.num_gates = len(circuit.gates)
 
     def run(self, state: torch.Tensor) -> torch.Tensor:
         """Apply the program to a contiguous (batch, 2**n) state in place."""

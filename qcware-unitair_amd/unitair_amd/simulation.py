@@ -138,8 +138,67 @@ def _op_for_batch(op: torch.Tensor, op_batch: Tuple[int, ...], sb: Tuple[int, ..
     return o, D * D
 
 
+def _needs_grad(*ts) -> bool:
+    return torch.is_grad_enabled() and any(t.requires_grad for t in ts)
+
+
+class _ApplyDense(torch.autograd.Function):
+    """out[b] = op[b or 0] on `qubits` of state[b or 0].
+
+    Backward (derivation: out = U x on each group of amplitudes):
+      grad_state = U^H grad_out          (ua_apply_operator, same kernel)
+      grad_op    = sum grad_out x^H      (ua_operator_gradient)
+    """
+
+    @staticmethod
+    def forward(ctx, op, state, qubits, n, k, out_batch, expand_state, op_bstride):
+        ctx.save_for_backward(op, state)
+        ctx.meta = (tuple(qubits), n, k, tuple(out_batch), expand_state, op_bstride)
+        return _launch_operator(op, qubits, state, n, k, out_batch, expand_state, op_bstride)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        op, state = ctx.saved_tensors
+        qubits, n, k, out_batch, expand_state, op_bstride = ctx.meta
+        g = grad_out.contiguous()
+        grad_op = grad_state = None
+        if ctx.needs_input_grad[1]:
+            op_h = op.conj().transpose(-1, -2).contiguous()
+            full = _launch_operator(op_h, qubits, g, n, k, out_batch, False, op_bstride)
+            grad_state = full.reshape(-1, 2 ** n).sum(0).reshape(state.shape) \
+                if expand_state else full.reshape(state.shape)
+        if ctx.needs_input_grad[0]:
+            grad_op = _operator_gradient(g, state, op, qubits, n, k, out_batch, expand_state,
+                                         op_bstride)
+        return grad_op, grad_state, None, None, None, None, None, None
+
+
+def _operator_gradient(g, state, op, qubits, n, k, out_batch, expand_state, op_bstride):
+    lib = _lib.load()
+    batch = math.prod(out_batch)
+    N = 2 ** n
+    grad = torch.empty_like(op)
+    if batch == 0:
+        return grad.zero_()
+    if k > _lib.UA_MAX_GRAD_QUBITS:
+        raise NotImplementedError(f"operator gradients support up to "
+                                  f"{_lib.UA_MAX_GRAD_QUBITS} qubits")
+    s_flat, s_stride = (state.reshape(N), 0) if expand_state else _flat_batch(state, N)
+    op_count = 1 if op_bstride == 0 else batch
+    with on_device(state.device):
+        rc = lib.ua_operator_gradient(
+            _lib.ptr(grad), _lib.ptr(g), _lib.ptr(s_flat.contiguous() if expand_state else s_flat),
+            _lib.DTYPE_CODES[state.dtype], n, k, _lib.int_array(qubits), batch, op_count,
+            s_stride, _lib.stream_handle(state.device))
+    _lib.check(rc, "operator gradient")
+    return grad
+
+
 def _launch_operator(op, qubits, state, n, k, out_batch, expand_state, op_bstride):
     lib = _lib.load()
+    op, state = prepare(op), prepare(state)
+    if not op.is_contiguous():
+        op = op.contiguous()
     N = 2 ** n
     if expand_state:
         s_flat, s_stride = state.reshape(N), 0
@@ -174,6 +233,11 @@ def _apply_dense(operator: torch.Tensor, qubits: List[int], state: torch.Tensor,
         k = len(qubits)
     D = 2 ** k
     op, op_bstride = _op_for_batch(operator, op_batch, sb, D)
+    if _needs_grad(op, state):
+        if not expand_state:
+            state = state.contiguous()
+        return _ApplyDense.apply(op, state, list(qubits), n, k, tuple(sb), expand_state,
+                                 op_bstride)
     return _launch_operator(op, qubits, state, n, k, sb, expand_state, op_bstride)
 
 
@@ -230,6 +294,13 @@ def apply_all_qubits(operator: torch.Tensor, state: torch.Tensor) -> torch.Tenso
             f"batch {sbatch} that would be matched against qubit dims")
     operator = operator.reshape(op_batch + (2, 2))
     op, op_bstride = _op_for_batch(operator, op_batch, sb, 2)
+    if _needs_grad(op, state):
+        # differentiable form: one dense application per qubit
+        x = state
+        for q in range(n):
+            x = _ApplyDense.apply(op, x if (q > 0 or not expand_state) else state, [q], n, 1,
+                                  tuple(sb), expand_state and q == 0, op_bstride)
+        return x
     lib = _lib.load()
     N = 2 ** n
     if expand_state:
@@ -271,6 +342,34 @@ def _coalesce(shape: Sequence[int], *strides: Sequence[int]):
     return shape2, strides2
 
 
+class _ApplyPhase(torch.autograd.Function):
+    """exp(-1j*angles) * state.  Backward (torch's conjugate-Wirtinger
+    convention): grad_state = exp(+1j*conj(angles)) * grad_out, reduced to the
+    state's shape; grad_angles = i*conj(out)*grad_out (its real part,
+    Im(conj(grad_out)*out), for real angles), reduced to the angles' shape."""
+
+    @staticmethod
+    def forward(ctx, angles, state):
+        out = _phase_launch(angles, state)
+        ctx.save_for_backward(angles, state, out)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        angles, state, out = ctx.saved_tensors
+        ga = gs = None
+        if ctx.needs_input_grad[1]:
+            back = _phase_launch(-angles.conj() if angles.is_complex() else -angles, g)
+            gs = back.sum_to_size(state.shape)
+            gs = gs if state.is_complex() else gs.real
+            gs = gs.to(state.dtype)
+        if ctx.needs_input_grad[0]:
+            full = 1j * out.conj() * g
+            ga = full.sum_to_size(angles.shape)
+            ga = (ga if angles.is_complex() else ga.real).to(angles.dtype)
+        return ga, gs
+
+
 def apply_phase(angles: torch.Tensor, state: torch.Tensor) -> torch.Tensor:
     """exp(-1j * angles) * state with torch broadcasting (operations.py:15-42).
 
@@ -294,6 +393,13 @@ def apply_phase(angles: torch.Tensor, state: torch.Tensor) -> torch.Tensor:
         angles = angles.to(torch.float32)      # ints/bool: exp(-1j*x) is complex64
     if state.dtype not in _PHASE_OF:
         state = state.to(torch.float32)
+    if _needs_grad(angles, state):
+        return _ApplyPhase.apply(angles, state)
+    return _phase_launch(angles, state)
+
+
+def _phase_launch(angles: torch.Tensor, state: torch.Tensor) -> torch.Tensor:
+    angles, state = prepare(angles), prepare(state)
     out_dtype = torch.promote_types(_PHASE_OF[angles.dtype], state.dtype)
     shape = torch.broadcast_shapes(angles.shape, state.shape)
     out = torch.empty(shape, dtype=out_dtype, device=state.device)
