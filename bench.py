@@ -36,6 +36,8 @@ def parse():
                    help="target CPU time of the cpu_baseline sample (0 = skip)")
     p.add_argument("--no-fidelity", action="store_true")
     p.add_argument("--no-out-of-place", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--no-serving", action="store_true")
     return p.parse_args()
 
 
@@ -121,6 +123,101 @@ def cpu_baseline(circ, n, seed, budget_s):
                       f"complex64 circuit, torch {torch.__version__} CPU restatement of the "
                       f"reference op sequence (permute/contiguous/einsum/exp), "
                       f"{t_used:.1f} s"}
+
+
+def adjoint_bench(dev, iters=3):
+    """16 qubits, 1024 states, 10 layers of RZ.RY on every qubit + CZ ring,
+    loss sum_i <Z_i>: forward + adjoint backward (complex64), against the
+    plain per-gate path (unitair_amd.simulation.apply_operator per gate with
+    torch autograd, one saved state per gate) at a batch that fits memory."""
+    import numpy as np
+    import torch
+    from unitair_amd import adjoint
+    from unitair_amd import simulation as sim
+    n, L, B = 16, 10, 1024
+    rng = np.random.default_rng(3)
+    th = torch.from_numpy(rng.uniform(0, 2 * np.pi, (B, L, n))).to(dev, torch.float32)
+    ph = torch.from_numpy(rng.uniform(0, 2 * np.pi, (B, L, n))).to(dev, torch.float32)
+
+    def adj_step(t, p):
+        t = t.detach().requires_grad_(True)
+        p = p.detach().requires_grad_(True)
+        adjoint.sum_z_loss(t, p, n, dtype=torch.complex64).sum().backward()
+        return t.grad, p.grad
+
+    adj_step(th, ph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        adj_step(th, ph)
+    torch.cuda.synchronize()
+    adj_s = (time.perf_counter() - t0) / iters
+
+    Bp = 64
+    pop = torch.tensor([bin(v).count("1") for v in range(2 ** n)], device=dev,
+                       dtype=torch.float32)
+    diag = n - 2 * pop
+    cz = torch.diag(torch.tensor([1, 1, 1, -1], dtype=torch.complex64, device=dev))
+
+    def plain_step(t, p):
+        t = t.detach().requires_grad_(True)
+        p = p.detach().requires_grad_(True)
+        psi = torch.zeros(Bp, 2 ** n, dtype=torch.complex64, device=dev)
+        psi[:, 0] = 1
+        for l in range(L):
+            for q in range(n):
+                e = torch.polar(torch.ones_like(p[:, l, q]), -p[:, l, q] / 2)
+                z = torch.zeros_like(e)
+                rz = torch.stack([torch.stack([e, z], -1), torch.stack([z, e.conj()], -1)], -2)
+                psi = sim.apply_operator(rz, [q], psi)
+            for q in range(n):
+                c, s_ = torch.cos(t[:, l, q] / 2), torch.sin(t[:, l, q] / 2)
+                ry = torch.stack([torch.stack([c, -s_], -1), torch.stack([s_, c], -1)], -2)
+                psi = sim.apply_operator(ry.to(torch.complex64), [q], psi)
+            for q in range(n):
+                psi = sim.apply_operator(cz, [q, (q + 1) % n], psi)
+        ((psi.abs() ** 2) * diag).sum().backward()
+        return t.grad, p.grad
+
+    plain_step(th[:Bp], ph[:Bp])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_step(th[:Bp], ph[:Bp])
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    return {"workload": f"{n} qubits, {L} layers (RZ.RY on every qubit + CZ ring), loss "
+                        f"sum_i <Z_i>, complex64, forward + gradient of all {2 * L * n} "
+                        f"angles per state",
+            "adjoint_batch": B, "adjoint_ms": adj_s * 1e3,
+            "adjoint_states_per_s": B / adj_s,
+            "plain_per_gate_autograd_batch": Bp, "plain_per_gate_autograd_ms": plain_s * 1e3,
+            "plain_per_gate_autograd_states_per_s": Bp / plain_s,
+            "speedup_states_per_s": (B / adj_s) / (Bp / plain_s)}
+
+
+def serving_bench(dev, num=24, seed=11):
+    """Poisson arrivals of random layered circuits (n log-uniform in [10, 26],
+    depth uniform in [10, 200], complex64).  The arrival rate is 0.7 x the
+    one-at-a-time capacity measured on the same requests; the continuous-
+    batching server and the one-at-a-time plain path see identical arrivals."""
+    from unitair_amd.serving import Server, poisson_workload, run_one_at_a_time, summarize
+    warm = poisson_workload(4, rate=0.0, seed=seed + 1, n_range=(10, 14), depth_range=(10, 20))
+    Server(device=dev).run(warm, realtime=False)
+    run_one_at_a_time(poisson_workload(4, 0.0, seed + 1, (10, 14), (10, 20)), device=dev,
+                      realtime=False)
+    cap_reqs = poisson_workload(num, rate=0.0, seed=seed)
+    run_one_at_a_time(cap_reqs, device=dev, realtime=False)
+    capacity = num / max(r.done for r in cap_reqs)
+    rate = 0.7 * capacity
+    served = Server(device=dev).run(poisson_workload(num, rate=rate, seed=seed), realtime=True)
+    plain = run_one_at_a_time(poisson_workload(num, rate=rate, seed=seed), device=dev,
+                              realtime=True)
+    same = max(max(abs(a.z0 - b.z0), abs(a.norm2 - b.norm2)) for a, b in zip(served, plain))
+    return {"workload": f"{num} requests, n log-uniform [10,26], depth U[10,200] layers of "
+                        f"Haar 1q on every qubit + CZ on n//2 adjacent pairs, complex64",
+            "one_at_a_time_capacity_circuits_per_s": capacity, "arrival_rate_per_s": rate,
+            "continuous_batching": summarize(served), "one_at_a_time": summarize(plain),
+            "max_result_difference": same}
 
 
 def fidelity(circ, n, seed):
@@ -211,6 +308,22 @@ def main():
     extra["step_algorithmic_GBps"] = step_bytes / sec / 1e9
     extra["step_frac_of_hbm_peak"] = extra["step_algorithmic_GBps"] / HBM_PEAK_GBS
     extra["kernels"] = kernels
+
+    if not args.no_adjoint:
+        log("adjoint gradient workload")
+        extra["adjoint"] = adjoint_bench(dev)
+        log(f"adjoint: {extra['adjoint']['adjoint_ms']:.1f} ms for 1024 states; plain "
+            f"per-gate autograd {extra['adjoint']['plain_per_gate_autograd_ms']:.1f} ms for 64")
+
+    if not args.no_serving:
+        log("serving workload")
+        extra["serving"] = serving_bench(dev)
+        sv = extra["serving"]
+        log(f"serving: continuous p50 {sv['continuous_batching']['p50_ms']:.1f} ms p99 "
+            f"{sv['continuous_batching']['p99_ms']:.1f} ms, "
+            f"{sv['continuous_batching']['circuits_per_s']:.1f} circ/s; one-at-a-time p50 "
+            f"{sv['one_at_a_time']['p50_ms']:.1f} p99 {sv['one_at_a_time']['p99_ms']:.1f} ms, "
+            f"{sv['one_at_a_time']['circuits_per_s']:.1f} circ/s")
 
     if not args.no_fidelity:
         log("fidelity vs complex128 oracle")

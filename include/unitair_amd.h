@@ -225,8 +225,11 @@ enum {
   UA_FOP_CNOT = 2,   /* a: control state index bit, b: target cube bit       */
   UA_FOP_CZ = 3,     /* a, b: state index bits                               */
   UA_FOP_DIAG1 = 4,  /* a: state index bit; data: (d0, d1), state dtype      */
-  UA_FOP_PHASE = 5   /* data: 2**n real angles, exp(-1j*angle) per element
+  UA_FOP_PHASE = 5,  /* data: 2**n real angles, exp(-1j*angle) per element
                         (float32 for complex64 states, float64 for complex128) */
+  UA_FOP_CZ_LAYER = 6 /* data: uint64 mask(s); bit j set = CZ between index bits
+                        j and j+1 (qubits n-2-j and n-1-j); data_bstride in
+                        masks (1 = one mask per state) */
 };
 
 #define UA_FUSED_MATRIX_CACHE 1024 /* elements of gate data a pass keeps on chip */
@@ -245,7 +248,7 @@ typedef struct {
   int32_t tile_bits;           /* m in 4..12, m <= num_qubits */
   int32_t local_bits[12];      /* state bit of tile bit i: ascending, [0..4) = 0,1,2,3 */
   int32_t num_stages;          /* <= UA_FUSED_MAX_STAGES */
-  int32_t needs_index;         /* 1 if any op is CNOT/CZ/DIAG1/PHASE */
+  int32_t needs_index;         /* 1 if any op is CNOT/CZ/DIAG1/PHASE/CZ_LAYER */
   int32_t stage_bits[UA_FUSED_MAX_STAGES][4]; /* ascending tile bits of each cube */
   int32_t stage_ops[UA_FUSED_MAX_STAGES + 1]; /* stage s runs ops [stage_ops[s], stage_ops[s+1]) */
 } ua_fused_pass;

@@ -297,6 +297,18 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
             for (int j = 0; j < kCube; ++j) c[j] = cmul(((gidx(j) >> op.a) & 1) ? d1 : d0, c[j]);
             break;
           }
+          case UA_FOP_CZ_LAYER: {  // data: one uint64 mask per state
+            const uint64_t mask = static_cast<const uint64_t*>(op.data)[b * op.data_bstride];
+#pragma unroll
+            for (int j = 0; j < kCube; ++j) {
+              const uint64_t g = gidx(j);
+              if (__popcll(g & (g >> 1) & mask) & 1) {
+                c[j].x = -c[j].x;
+                c[j].y = -c[j].y;
+              }
+            }
+            break;
+          }
           case UA_FOP_PHASE: {  // data: 2^n real angles per state
             const R* th = static_cast<const R*>(op.data) + b * op.data_bstride;
 #pragma unroll
@@ -360,7 +372,7 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
     const int nops = pass->stage_ops[pass->num_stages];
     for (int o = 0; o < nops; ++o) {
       const ua_fused_op& op = host_ops[o];
-      UA_CHECK_ARG(op.kind >= UA_FOP_DENSE1 && op.kind <= UA_FOP_PHASE, "op %d: bad kind", o);
+      UA_CHECK_ARG(op.kind >= UA_FOP_DENSE1 && op.kind <= UA_FOP_CZ_LAYER, "op %d: bad kind", o);
       if (op.kind == UA_FOP_DENSE1 || op.kind == UA_FOP_CNOT)
         UA_CHECK_ARG(op.a >= 0 && op.a < (op.kind == UA_FOP_CNOT ? n : 4) &&
                          (op.kind == UA_FOP_DENSE1 || (op.b >= 0 && op.b < 4)),
@@ -374,6 +386,7 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
       if (op.cache_slot >= 0) {
         const int len = op.kind == UA_FOP_DENSE1 ? 4 : op.kind == UA_FOP_DENSE2 ? 16 : 2;
         UA_CHECK_ARG(op.kind != UA_FOP_CZ && op.kind != UA_FOP_CNOT && op.kind != UA_FOP_PHASE &&
+                         op.kind != UA_FOP_CZ_LAYER &&
                          op.data_bstride == 0 && op.cache_slot + len <= kMatCache,
                      "op %d: bad cache slot %d", o, op.cache_slot);
       }

@@ -97,14 +97,22 @@ class CircuitRunner:
         real = _REAL[dtype]
         gates = (_lib.Gate * max(1, len(circuit.gates)))()
         kinds = []
+        # all dense matrices go to the device in one copy
+        dense = [np.asarray(g.matrix, dtype=np.complex128).reshape(-1)
+                 for g in circuit.gates if g.kind == "dense"]
+        pool = None
+        if dense:
+            pool = torch.from_numpy(np.concatenate(dense)).to(self.device, dtype)
+            self._buffers.append(pool)
+        esize = pool.element_size() if pool is not None else 0
+        offset = 0
         for i, g in enumerate(circuit.gates):
             c = gates[i]
             if g.kind == "dense":
                 if not 1 <= len(g.qubits) <= _lib.UA_MAX_GATE_QUBITS:
                     raise ValueError(f"dense gates act on 1..{_lib.UA_MAX_GATE_QUBITS} qubits")
-                m = torch.as_tensor(np.ascontiguousarray(g.matrix)).to(self.device, dtype)
-                self._buffers.append(m)
-                c.kind, c.data = _lib.UA_GATE_DENSE, m.data_ptr()
+                c.kind, c.data = _lib.UA_GATE_DENSE, pool.data_ptr() + offset * esize
+                offset += 4 ** len(g.qubits)
                 kinds.append(f"dense{len(g.qubits)}q")
             elif g.kind == "phase":
                 a = torch.as_tensor(np.ascontiguousarray(g.angles)).to(self.device, real)

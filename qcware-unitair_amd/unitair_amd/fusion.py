@@ -29,7 +29,7 @@ TILE_BITS = 12
 CUBE_BITS = 4
 MAX_STAGES = _lib.UA_FUSED_MAX_STAGES
 
-FOP_DENSE1, FOP_DENSE2, FOP_CNOT, FOP_CZ, FOP_DIAG1, FOP_PHASE = range(6)
+FOP_DENSE1, FOP_DENSE2, FOP_CNOT, FOP_CZ, FOP_DIAG1, FOP_PHASE, FOP_CZ_LAYER = range(7)
 
 
 FusedOp = _lib.FusedOp
@@ -38,7 +38,8 @@ FusedPass = _lib.FusedPass
 
 @dataclass
 class _Item:
-    """A gate in state-bit terms.  kind: dense1/dense2/cnot/cz/phase/diag1."""
+    """A gate in state-bit terms.  kind: dense1/dense2/cnot/cz/phase/diag1/
+    cz_layer (data: int64 mask(s) of adjacent index-bit pairs)."""
     kind: str
     bits: tuple                      # state bits (dense: gate MSB first)
     data: Optional[np.ndarray] = None
@@ -57,7 +58,7 @@ def _merge_1q(items: List[_Item]) -> List[_Item]:
     out: List[_Item] = []
     last_on_bit = {}
     for it in items:
-        touched = set(it.bits) if it.kind not in ("phase",) else None
+        touched = set(it.bits) if it.kind not in ("phase", "cz_layer") else None
         if it.kind == "dense1" and not it.per_state:
             j = last_on_bit.get(it.bits[0])
             if j is not None and out[j].kind == "dense1" and not out[j].per_state:
@@ -246,6 +247,9 @@ class FusedProgram:
                         elif it.kind == "phase":
                             o.kind = FOP_PHASE
                             needs_index = 1
+                        elif it.kind == "cz_layer":
+                            o.kind = FOP_CZ_LAYER
+                            needs_index = 1
                         else:
                             raise ValueError(it.kind)
                         if it.data is not None:
@@ -259,7 +263,7 @@ class FusedProgram:
         # upload operand data and fill data pointers
         for idx, it in pending:
             o = op_structs[idx]
-            want = real if it.kind == "phase" else dtype
+            want = {"phase": real, "cz_layer": torch.int64}.get(it.kind, dtype)
             if isinstance(it.data, torch.Tensor):
                 t = it.data
                 if t.device != self.device or t.dtype != want or not t.is_contiguous():
@@ -267,7 +271,8 @@ class FusedProgram:
                                      "dtype and device")
             else:
                 t = torch.as_tensor(np.ascontiguousarray(it.data)).to(self.device, want)
-            per = {"phase": 2 ** self.n, "diag1": 2, "dense1": 4, "dense2": 16}[it.kind]
+            per = {"phase": 2 ** self.n, "diag1": 2, "dense1": 4, "dense2": 16,
+                   "cz_layer": 1}[it.kind]
             o.data_bstride = per if it.per_state else 0
             self._keep.append(t)
             o.data = t.data_ptr()

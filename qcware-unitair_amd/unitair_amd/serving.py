@@ -1,0 +1,338 @@
+"""Continuous-batching request loop for circuits of uneven size and depth.
+
+A request is a seeded random circuit on n qubits (complex64): `depth`
+layers, each a Haar-random 1-qubit gate on every qubit followed by CZ on
+floor(n/2) random adjacent pairs; the answer is ||psi||^2 and <Z_0> in
+float64.  (unitair itself has no serving layer: a user loops over
+apply_operator calls; README.rst:11-16.)
+
+Design (one GPU):
+  * Admission by footprint.  Same-n requests share a lane: one HBM buffer of
+    `capacity` state slots (capacity x 2^n x 8 B).  A request is admitted
+    when its lane has a free slot, or when a new lane fits the HBM budget;
+    otherwise it waits in the queue.  Lanes are released when they empty.
+  * Continuous batching.  Every scheduler step advances each lane by
+    K = min(max_layers_per_step, layers left of its least advanced request)
+    layers, with one batched fused-pass program over the lane's occupied
+    slot range: per-slot gate data (each slot is at its own layer), so
+    requests that arrived later join the next step of their lane.  Empty
+    slots inside the range run identity gates.
+  * Overlap.  Each lane runs on its own HIP stream, so lanes of different n
+    overlap on the device; the step's gate data is staged once per lane.
+  * Graph replay.  Each (lane, K, occupied range) program is captured into a
+    HIP graph on first use and replayed afterwards.
+Each request's arithmetic is independent of its slot and of the other
+requests, so its result is the same alone and in the mix.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._device import on_device
+from .fusion import FusedProgram, _Item
+from .gates import haar_unitary
+
+BYTES_C64 = 8
+
+
+@dataclass
+class Request:
+    rid: int
+    n: int
+    depth: int
+    seed: int
+    arrival: float = 0.0          # seconds after the start of the run
+    # filled by the server
+    norm2: Optional[float] = None
+    z0: Optional[float] = None
+    admitted: Optional[float] = None
+    done: Optional[float] = None
+
+    @property
+    def footprint(self) -> int:
+        return (2 ** self.n) * BYTES_C64
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+    def gate_data(self):
+        """(depth, n, 2, 2) complex64 Haar gates, (depth,) int64 CZ masks.
+        Mask bit j: CZ between index bits j, j+1, i.e. qubits (n-2-j, n-1-j)."""
+        rng = np.random.default_rng(self.seed)
+        mats = np.empty((self.depth, self.n, 2, 2), dtype=np.complex64)
+        masks = np.zeros(self.depth, dtype=np.int64)
+        for l in range(self.depth):
+            for q in range(self.n):
+                mats[l, q] = haar_unitary(2, rng)
+            pairs = rng.choice(self.n - 1, size=self.n // 2, replace=False)
+            for i in pairs:                       # pair (i, i+1) in qubit order
+                masks[l] |= 1 << (self.n - 2 - int(i))
+        return mats, masks
+
+    def circuit(self):
+        """The same request as a unitair_amd Circuit (one gate per call)."""
+        from .circuits import Circuit, Gate
+        mats, masks = self.gate_data()
+        gates = []
+        for l in range(self.depth):
+            for q in range(self.n):
+                gates.append(Gate("dense", (q,), matrix=mats[l, q].astype(np.complex128)))
+            m = int(masks[l])
+            for j in range(self.n - 1):
+                if (m >> j) & 1:
+                    gates.append(Gate("cz", (self.n - 2 - j, self.n - 1 - j)))
+        return Circuit(self.n, gates)
+
+
+def poisson_workload(num: int, rate: float, seed: int = 5, n_range=(10, 26),
+                     depth_range=(10, 200)) -> List[Request]:
+    """Requests with exponential inter-arrival times (rate per second), n
+    log-uniform over n_range (inclusive), depth uniform over depth_range."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    lo, hi = n_range
+    for i in range(num):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        n = int(math.floor(math.exp(rng.uniform(math.log(lo), math.log(hi + 1)))))
+        n = min(max(n, lo), hi)
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        reqs.append(Request(i, n, depth, seed=int(rng.integers(2 ** 62)), arrival=t))
+    return reqs
+
+
+def _z0_diag(n: int, device) -> torch.Tensor:
+    """1 - 2*(bit of qubit 0), qubit 0 = index bit n-1."""
+    x = torch.arange(2 ** n, device=device)
+    return (1 - 2 * ((x >> (n - 1)) & 1)).to(torch.float64)
+
+
+class _Lane:
+    def __init__(self, n: int, capacity: int, kmax: int, device):
+        self.n = n
+        self.capacity = capacity
+        self.kmax = kmax
+        self.device = device
+        self.states = torch.empty(capacity, 2 ** n, dtype=torch.complex64, device=device)
+        self.mats = torch.empty(kmax, n, capacity, 2, 2, dtype=torch.complex64, device=device)
+        self.masks = torch.zeros(kmax, capacity, dtype=torch.int64, device=device)
+        self.h_mats = torch.empty(self.mats.shape, dtype=torch.complex64, pin_memory=True)
+        self.h_masks = torch.zeros(self.masks.shape, dtype=torch.int64, pin_memory=True)
+        self.slots: List[Optional[Request]] = [None] * capacity
+        self.pos = [0] * capacity           # next layer of each slot
+        self.data: Dict[int, tuple] = {}    # rid -> (mats, masks) host arrays
+        self.stream = torch.cuda.Stream(device)   # default priority
+        self.programs: Dict[tuple, tuple] = {}
+        self.diag = _z0_diag(n, device)
+        self.identity = np.eye(2, dtype=np.complex64)
+
+    @property
+    def bytes(self) -> int:
+        return self.capacity * (2 ** self.n) * BYTES_C64
+
+    def free_slot(self) -> Optional[int]:
+        for i, r in enumerate(self.slots):
+            if r is None:
+                return i
+        return None
+
+    def active(self):
+        return [i for i, r in enumerate(self.slots) if r is not None]
+
+    def _program(self, K: int, used: int, use_graph: bool):
+        key = (K, used)
+        if key not in self.programs:
+            items = []
+            for l in range(K):
+                for q in range(self.n):
+                    items.append(_Item("dense1", (self.n - 1 - q,), self.mats[l, q], True))
+                items.append(_Item("cz_layer", (), self.masks[l], True))
+            prog = FusedProgram.from_items(items, self.n, torch.complex64, self.device)
+            # data_bstride for per-slot matrices: consecutive slots are 4 (2x2)
+            # elements apart inside mats[l, q] (shape (capacity, 2, 2)), and 1
+            # mask apart inside masks[l] -- both match the defaults.
+            graph = None
+            if use_graph:
+                view = self.states[:used]
+                with torch.cuda.stream(self.stream):
+                    prog.capture(view)
+                graph = prog
+            self.programs[key] = (prog, graph)
+        return self.programs[key]
+
+    def step(self, use_graph: bool):
+        act = self.active()
+        used = max(act) + 1
+        K = min(self.kmax, min(self.slots[i].depth - self.pos[i] for i in act))
+        for i in range(used):
+            r = self.slots[i]
+            if r is None:
+                self.h_mats[:K, :, i] = torch.from_numpy(self.identity)
+                self.h_masks[:K, i] = 0
+                continue
+            mats, masks = self.data[r.rid]
+            p = self.pos[i]
+            self.h_mats[:K, :, i] = torch.from_numpy(mats[p:p + K])
+            self.h_masks[:K, i] = torch.from_numpy(masks[p:p + K])
+        with torch.cuda.stream(self.stream):
+            self.mats.copy_(self.h_mats, non_blocking=True)
+            self.masks.copy_(self.h_masks, non_blocking=True)
+            prog, graph = self._program(K, used, use_graph)
+            if graph is not None:
+                graph.replay()
+            else:
+                prog.run(self.states[:used])
+        for i in act:
+            self.pos[i] += K
+        return K
+
+    def finished(self):
+        return [i for i in self.active() if self.pos[i] >= self.slots[i].depth]
+
+    def collect(self, i: int):
+        """float64 (||psi||^2, <Z_0>) of slot i, computed on the lane stream."""
+        from .states import _reduce
+        with torch.cuda.stream(self.stream):
+            row = self.states[i:i + 1]
+            norm = _reduce("norm", torch.float64, row)
+            z0 = _reduce("diag", torch.float64, self.diag, row)
+        return norm, z0
+
+
+class Server:
+    """Continuous-batching executor on one GPU (see the module docstring)."""
+
+    def __init__(self, hbm_budget: int = 64 << 30, max_slots: int = 32,
+                 max_layers_per_step: int = 8, device="cuda", use_graphs: bool = True,
+                 graph_max_qubits: int = 20):
+        self.budget = hbm_budget
+        self.max_slots = max_slots
+        self.kmax = max_layers_per_step
+        self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.use_graphs = use_graphs
+        self.graph_max_qubits = graph_max_qubits
+        self.lanes: Dict[int, _Lane] = {}
+        self.used_bytes = 0
+        self.log: List[str] = []
+
+    def _lane_capacity(self, n: int) -> int:
+        fp = (2 ** n) * BYTES_C64
+        return int(max(1, min(self.max_slots, (self.budget // 4) // fp)))
+
+    def try_admit(self, r: Request, now: float) -> bool:
+        if r.footprint > self.budget:
+            raise ValueError(f"request {r.rid}: {r.footprint} B exceeds the HBM budget")
+        lane = self.lanes.get(r.n)
+        if lane is None:
+            cap = self._lane_capacity(r.n)
+            need = cap * r.footprint
+            if self.used_bytes + need > self.budget:
+                cap = (self.budget - self.used_bytes) // r.footprint
+                if cap < 1:
+                    return False
+                need = cap * r.footprint
+            with on_device(self.device):
+                lane = _Lane(r.n, int(cap), self.kmax, self.device)
+            self.lanes[r.n] = lane
+            self.used_bytes += need
+        slot = lane.free_slot()
+        if slot is None:
+            return False
+        lane.slots[slot] = r
+        lane.pos[slot] = 0
+        lane.data[r.rid] = r.gate_data()
+        with torch.cuda.stream(lane.stream):
+            lane.states[slot].zero_()
+            lane.states[slot, 0] = 1
+        r.admitted = now
+        return True
+
+    def run(self, requests: List[Request], realtime: bool = True) -> List[Request]:
+        """Serve `requests` (arrival times in seconds from now).  Returns them
+        with results and completion times filled in."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting: List[Request] = []
+        done: List[Request] = []
+        t0 = time.perf_counter()
+        i = 0
+        with on_device(self.device):
+            while len(done) < len(requests):
+                now = time.perf_counter() - t0 if realtime else float("inf")
+                while i < len(pending) and pending[i].arrival <= now:
+                    waiting.append(pending[i])
+                    i += 1
+                still = []
+                for r in waiting:                 # arrival order
+                    if not self.try_admit(r, now):
+                        still.append(r)
+                waiting = still
+                busy = [l for l in self.lanes.values() if l.active()]
+                if not busy:
+                    if i < len(pending) and realtime:
+                        time.sleep(max(0.0, min(0.001, pending[i].arrival - now)))
+                    elif waiting:
+                        raise RuntimeError("requests cannot be admitted under the budget")
+                    continue
+                for lane in busy:
+                    lane.step(self.use_graphs and lane.n <= self.graph_max_qubits)
+                results = []
+                for lane in busy:
+                    for s in lane.finished():
+                        results.append((lane, s, lane.slots[s]) + lane.collect(s))
+                torch.cuda.synchronize(self.device)
+                tnow = time.perf_counter() - t0
+                for lane, s, r, norm, z0 in results:
+                    r.norm2 = float(norm)
+                    r.z0 = float(z0)
+                    r.done = tnow
+                    lane.slots[s] = None
+                    del lane.data[r.rid]
+                    done.append(r)
+                for n in [n for n, l in self.lanes.items() if not l.active()]:
+                    self.used_bytes -= self.lanes[n].bytes
+                    del self.lanes[n]
+        return requests
+
+
+def run_one_at_a_time(requests: List[Request], device="cuda", realtime: bool = True):
+    """Plain GPU execution: requests in arrival order, one gate per launch
+    (ua_run_circuit), one request at a time.  Returns the requests with
+    results and completion times."""
+    from .circuits import CircuitRunner
+    from .states import norm_squared_f64, _reduce
+    dev = torch.device(device)
+    t0 = time.perf_counter()
+    for r in sorted(requests, key=lambda r: r.arrival):
+        if realtime:
+            while time.perf_counter() - t0 < r.arrival:
+                time.sleep(0.0005)
+        runner = CircuitRunner(r.circuit(), dtype=torch.complex64, device=dev)
+        psi = torch.zeros(1, 2 ** r.n, dtype=torch.complex64, device=dev)
+        psi[0, 0] = 1
+        runner.run(psi)
+        norm = norm_squared_f64(psi)
+        z0 = _reduce("diag", torch.float64, _z0_diag(r.n, dev), psi)
+        torch.cuda.synchronize(dev)
+        r.done = time.perf_counter() - t0
+        r.norm2, r.z0 = float(norm[0]), float(z0[0])
+    return requests
+
+
+def summarize(requests: List[Request]):
+    lat = np.array([r.latency for r in requests])
+    makespan = max(r.done for r in requests) - min(r.arrival for r in requests)
+    amps = sum(r.depth * (r.n + r.n // 2) * (2 ** r.n) for r in requests)
+    return {"requests": len(requests), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "makespan_s": makespan,
+            "circuits_per_s": len(requests) / makespan,
+            "amp_updates_per_s": amps / makespan}
