@@ -61,8 +61,13 @@ const char* ua_last_error(void);
  *                       when one operator serves the whole batch
  * UA_TUNE_LANE_EXCHANGE 1 (default): 1-qubit gates on index bits 1..6 move
  *                       contiguous amplitude pairs and exchange partners
- *                       across lanes; 0: strided pair access. */
-enum { UA_TUNE_UNROLL = 0, UA_TUNE_LANE_EXCHANGE = 1 };
+ *                       across lanes; 0: strided pair access.
+ * UA_TUNE_MFMA          1 (default): complex gates with one operator for the
+ *                       batch run on the matrix cores (v_mfma_f32/f64_16x16x4)
+ *                       when k = 5, when k = 4 in complex128, and when k = 4
+ *                       in complex64 with every target on index bits 0..5;
+ *                       0: vector units for every gate. */
+enum { UA_TUNE_UNROLL = 0, UA_TUNE_LANE_EXCHANGE = 1, UA_TUNE_MFMA = 2 };
 int ua_set_tuning(int key, int value);
 
 /*

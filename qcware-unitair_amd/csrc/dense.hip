@@ -28,6 +28,10 @@
 namespace ua {
 
 int g_unroll = 0;      // units per thread for single-operator launches (0 = auto)
+extern bool g_use_mfma;  // 4/5-qubit complex gates on the matrix cores
+int apply_dense_mfma(void* out, const void* in, const void* op, int dtype, int n, int k,
+                     const int* tb, const int* gp, int64_t batch, int64_t in_bstride,
+                     hipStream_t s);
 bool g_use_xch = true; // lane-exchange path for 1-qubit gates on index bits 1..6
 
 namespace {
@@ -391,6 +395,16 @@ int apply_operator_impl(void* out, const void* state, const void* op, int dtype,
         int g0 = gp[j]; gp[j] = gp[j - 1]; gp[j - 1] = g0;
       }
     for (int m = 0; m < k; ++m) { p.tb[m] = tb[m]; p.gpos[m] = gp[m]; }
+    // Matrix cores for the fused-block sizes where they measured faster on
+    // MI355X (28-qubit complex64 / 27-qubit complex128 states): every 5-qubit
+    // gate; complex128 4-qubit gates; complex64 4-qubit gates only when all
+    // targets lie in the low 6 bits (the vector kernel streams the others at
+    // HBM speed).
+    const bool mfma_shape =
+        (k == 5 && dtype != UA_FLOAT32 && dtype != UA_FLOAT64) ||
+        (k == 4 && dtype == UA_COMPLEX128) || (k == 4 && dtype == UA_COMPLEX64 && tb[k - 1] < 6);
+    if (g_use_mfma && mfma_shape && op_bstride == 0 && n - k >= 6)
+      return apply_dense_mfma(out, state, op, dtype, n, k, tb, gp, batch, state_bstride, s);
 
     const size_t pair_align = 2 * es;
     const bool io_aligned = aligned(state, state_bstride, es, pair_align) &&
@@ -480,6 +494,9 @@ extern "C" int ua_set_tuning(int key, int value) {
       return UA_OK;
     case UA_TUNE_LANE_EXCHANGE:
       ua::g_use_xch = value != 0;
+      return UA_OK;
+    case UA_TUNE_MFMA:
+      ua::g_use_mfma = value != 0;
       return UA_OK;
   }
   ua::set_error("unknown tuning key %d", key);

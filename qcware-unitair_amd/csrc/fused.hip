@@ -250,6 +250,15 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
       return gbase | ((j & 1) ? g0 : 0) | ((j & 2) ? g1 : 0) | ((j & 4) ? g2 : 0) |
              ((j & 8) ? g3 : 0);
     };
+    // 16-bit mask of the cube entries whose state index has bit `bit` set
+    const int cb0 = a.local[s0], cb1 = a.local[s1], cb2 = a.local[s2], cb3 = a.local[s3];
+    auto sel = [&](int bit) -> uint32_t {
+      if (bit == cb0) return 0xAAAAu;
+      if (bit == cb1) return 0xCCCCu;
+      if (bit == cb2) return 0xF0F0u;
+      if (bit == cb3) return 0xFF00u;
+      return ((gbase >> bit) & 1) ? 0xFFFFu : 0u;
+    };
     if (active) {
       for (int o = a.stage_ops[st]; o < a.stage_ops[st + 1]; ++o) {
         const ua_fused_op op = ops[o];
@@ -267,9 +276,7 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
             break;
           }
           case UA_FOP_CNOT: {  // a: control state bit, b: target cube bit
-            uint32_t ctrl = 0;
-#pragma unroll
-            for (int j = 0; j < kCube; ++j) ctrl |= uint32_t((gidx(j) >> op.a) & 1) << j;
+            const uint32_t ctrl = sel(op.a);
             switch (op.b) {
               case 0: cube_cnot<0>(c, ctrl); break;
               case 1: cube_cnot<1>(c, ctrl); break;
@@ -279,10 +286,10 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
             break;
           }
           case UA_FOP_CZ: {  // a, b: state bits
+            const uint32_t neg = sel(op.a) & sel(op.b);
 #pragma unroll
             for (int j = 0; j < kCube; ++j) {
-              const uint64_t g = gidx(j);
-              if ((g >> op.a) & (g >> op.b) & 1) {
+              if ((neg >> j) & 1) {
                 c[j].x = -c[j].x;
                 c[j].y = -c[j].y;
               }
@@ -293,8 +300,9 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
             const T* d = op.cache_slot >= 0 ? mcache + op.cache_slot
                                             : static_cast<const T*>(op.data) + b * op.data_bstride;
             const T d0 = d[0], d1 = d[1];
+            const uint32_t one = sel(op.a);
 #pragma unroll
-            for (int j = 0; j < kCube; ++j) c[j] = cmul(((gidx(j) >> op.a) & 1) ? d1 : d0, c[j]);
+            for (int j = 0; j < kCube; ++j) c[j] = cmul(((one >> j) & 1) ? d1 : d0, c[j]);
             break;
           }
           case UA_FOP_CZ_LAYER: {  // data: one uint64 mask per state

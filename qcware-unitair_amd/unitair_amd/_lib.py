@@ -63,6 +63,10 @@ SIGNATURES = {
     "ua_apply_cnot": (_i32, [_p, _i32, _i32, _i32, _i32, _i64, _p]),
 }
 
+UA_TUNE_UNROLL = 0
+UA_TUNE_LANE_EXCHANGE = 1
+UA_TUNE_MFMA = 2
+
 UA_PAULI_Y = 1
 UA_PAULI_Z = 2
 UA_MAX_GRAD_QUBITS = 16

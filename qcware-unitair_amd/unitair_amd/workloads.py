@@ -90,3 +90,23 @@ def random_circuit_24q(seed: int = 2, n: int = 24, n_1q: int = 120, n_2q_each: i
         gates.append(Gate("phase", (), angles=SeededAngles(n, rng.integers(2 ** 62))))
     order = rng.permutation(len(gates))
     return Circuit(n, [gates[i] for i in order])
+
+
+def fused_dense_circuit(seed: int = 4, n: int = 28, n5: int = 64, n4: int = 16) -> Circuit:
+    """One state, Haar-random dense blocks: n5 32x32 unitaries on random
+    5-qubit subsets (the first two are qubits 0-4 and n-5..n-1, the largest
+    and the smallest strides) and n4 16x16 unitaries on random 4-qubit
+    subsets; shuffled.  The same builder with a smaller n gives the slice
+    used for the complex128 CPU comparison."""
+    rng = np.random.default_rng(seed)
+    gates = []
+    subsets5 = [tuple(range(5)), tuple(range(n - 5, n))]
+    while len(subsets5) < n5:
+        subsets5.append(tuple(int(q) for q in rng.choice(n, 5, replace=False)))
+    for qs in subsets5:
+        gates.append(Gate("dense", qs, matrix=haar_unitary(32, rng)))
+    for _ in range(n4):
+        qs = tuple(int(q) for q in rng.choice(n, 4, replace=False))
+        gates.append(Gate("dense", qs, matrix=haar_unitary(16, rng)))
+    order = rng.permutation(len(gates))
+    return Circuit(n, [gates[i] for i in order])
