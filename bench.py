@@ -38,6 +38,15 @@ def parse():
     p.add_argument("--no-out-of-place", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--no-serving", action="store_true")
+    p.add_argument("--no-fused-dense", action="store_true",
+                   help="skip the 28-qubit dense 4/5-qubit block workload")
+    p.add_argument("--no-graph", action="store_true", help="skip the graph-replay leg")
+    p.add_argument("--no-ops", action="store_true", help="skip the per-call drop-in leg")
+    p.add_argument("--no-fused", action="store_true", help="skip the fused-pass leg")
+    p.add_argument("--copy-calibration-mib", type=int, default=0,
+                   help="run one float32 multiply-by-one of this many MiB (read + write) "
+                        "before the timed steps, as the known-size kernel that scales "
+                        "FETCH_SIZE / WRITE_SIZE counter readings")
     return p.parse_args()
 
 
@@ -220,6 +229,195 @@ def serving_bench(dev, num=24, seed=11):
             "max_result_difference": same}
 
 
+MFMA_PEAK_TFLOPS = {"c64": 157.3, "c128": 78.6}   # dense FP32 / FP64 matrix-core peaks
+
+
+def fused_dense_bench(dev, steps=3, warmup=1, seed=4, n=28):
+    """One n-qubit state; 64 Haar 32x32 blocks on random 5-qubit subsets
+    (including qubits 0-4 and n-5..n-1) and 16 Haar 16x16 blocks, in place,
+    complex64 (FP32 matrix cores) and complex128 (FP64 matrix cores).  One
+    step = the 80 gates."""
+    import numpy as np
+    import torch
+    from unitair_amd.circuits import CircuitRunner
+    from unitair_amd.initializations import seeded_state
+    from unitair_amd.workloads import fused_dense_circuit
+    circ = fused_dense_circuit(seed=seed, n=n)
+    amps = 2 ** n
+    flops = sum(8 * 2 ** len(g.qubits) * amps for g in circ.gates)
+    out = {"workload": f"{n} qubits, {circ.counts()}: 64 Haar 32x32 (5-qubit) + 16 Haar 16x16 "
+                       f"(4-qubit) blocks on random subsets, one state, in place",
+           "flops_per_step": flops}
+    base = seeded_state(n, (), seed=seed + 10, dtype=torch.complex128, device=dev)
+    for dt, name in ((torch.complex64, "c64"), (torch.complex128, "c128")):
+        state = base.to(dt) if dt != torch.complex128 else base
+        es = state.element_size()
+        runner = CircuitRunner(circ, dtype=dt, device=dev, in_place=True)
+        sec, per_kind = time_steps(runner, state, steps, warmup, with_events=True)
+        step_bytes = len(circ.gates) * 2 * amps * es
+        peak = MFMA_PEAK_TFLOPS[name]
+        kernels = {}
+        for kind, v in per_kind.items():
+            k = int(kind[len("dense"):-1])
+            us = 1e3 * float(np.mean(v))
+            kflops = 8 * 2 ** k * amps
+            kernels[kind] = {"launches_per_step": len(v) // steps, "avg_us": us,
+                             "algorithmic_GBps": 2 * amps * es / us / 1e3,
+                             "TFLOPs": kflops / us / 1e6,
+                             "mfma_frac": kflops / us / 1e6 / peak}
+        out[name] = {"ms_per_step": sec * 1e3,
+                     "amp_updates_per_s": len(circ.gates) * amps / sec,
+                     "algorithmic_GBps": step_bytes / sec / 1e9,
+                     "hbm_frac": step_bytes / sec / 1e9 / HBM_PEAK_GBS,
+                     "TFLOPs": flops / sec / 1e12,
+                     "mfma_frac": flops / sec / 1e12 / peak,
+                     "roofline_ms": 1e3 * max(step_bytes / (HBM_PEAK_GBS * 1e9),
+                                              flops / (peak * 1e12)),
+                     "kernels": kernels}
+        log(f"fused dense {name}: {sec * 1e3:.2f} ms/step, "
+            f"{out[name]['algorithmic_GBps']:.0f} GB/s, {out[name]['TFLOPs']:.1f} TFLOP/s; "
+            + ", ".join(f"{k}={v['avg_us']:.0f}us" for k, v in kernels.items()))
+        del runner, state
+        torch.cuda.empty_cache()
+    del base
+    torch.cuda.empty_cache()
+    return out
+
+
+def fused_dense_slice_fidelity(dev, seed=4, slice_qubits=20):
+    """The dense-block builder at slice_qubits qubits, run on the GPU in
+    complex64 and complex128, against the complex128 oracle."""
+    import torch
+    from oracle import circuits as oc
+    from oracle import unitair_np as U
+    from unitair_amd.circuits import CircuitRunner
+    from unitair_amd.initializations import seeded_state_np
+    from unitair_amd.workloads import fused_dense_circuit
+    sl = fused_dense_circuit(seed=seed, n=slice_qubits)
+    psi = seeded_state_np(slice_qubits, (), seed=seed + 20)
+    ref = oc.run_closed_form(sl.gates, psi)
+    out = {"slice": f"same builder at {slice_qubits} qubits ({len(sl.gates)} gates)"}
+    for dt, name in ((torch.complex64, "c64"), (torch.complex128, "c128")):
+        r = CircuitRunner(sl, dtype=dt, device=dev, in_place=True)
+        got = r.run(torch.from_numpy(psi).to(dev, dt)).cpu().numpy()
+        out[name] = {"fidelity_error": float(U.fidelity_error(ref, got)),
+                     "max_abs_error": float(U.max_abs_error(ref, got))}
+    return out
+
+
+def _ops_inputs(n, seed, device, dtype):
+    """One n-qubit state, Haar gates and full-length float32 angles / diag."""
+    import numpy as np
+    import torch
+    from unitair_amd.gates import haar_unitary
+    from unitair_amd.initializations import seeded_state
+    rng = np.random.default_rng(seed)
+    psi = seeded_state(n, (), seed=seed, dtype=dtype, device=device)
+    other = seeded_state(n, (), seed=seed + 1, dtype=dtype, device=device)
+    u = {k: torch.from_numpy(haar_unitary(2 ** k, rng)).to(device, dtype) for k in (1, 2, 5)}
+    ang = torch.from_numpy(rng.uniform(0, 2 * np.pi, 2 ** n).astype(np.float32)).to(device)
+    diag = torch.from_numpy(rng.standard_normal(2 ** n).astype(np.float32)).to(device)
+    return psi, other, u, ang, diag
+
+
+def _op_table(n, sim, states, psi, other, u, ang, diag):
+    """(name, callable, algorithmic bytes) for each drop-in call, complex64."""
+    A, s = 2 ** n, 8
+    return [
+        ("apply_operator 1q, qubit 0 (stride 2^(n-1))",
+         lambda: sim.apply_operator(u[1], [0], psi), 2 * A * s),
+        (f"apply_operator 1q, qubit {n - 1} (stride 1)",
+         lambda: sim.apply_operator(u[1], [n - 1], psi), 2 * A * s),
+        ("apply_operator 2q Haar, qubits (5, 17)",
+         lambda: sim.apply_operator(u[2], [5, 17], psi), 2 * A * s),
+        ("apply_operator 5q Haar, qubits (0..4)",
+         lambda: sim.apply_operator(u[5], [0, 1, 2, 3, 4], psi), 2 * A * s),
+        ("act_first_qubits 2q Haar",
+         lambda: sim.act_first_qubits(u[2], psi), 2 * A * s),
+        ("apply_phase, full-length float32 angles",
+         lambda: sim.apply_phase(ang, psi), 2 * A * s + A * 4),
+        ("apply_all_qubits Haar 2x2",
+         lambda: sim.apply_all_qubits(u[1], psi), 2 * A * s),
+        ("norm_squared", lambda: states.norm_squared(psi), A * s),
+        ("abs_squared", lambda: states.abs_squared(psi), A * s + A * 4),
+        ("inner_product", lambda: states.inner_product(psi, other), 2 * A * s),
+        ("diag_expectation_value, float32 diag",
+         lambda: states.diag_expectation_value(diag, psi), A * s + A * 4),
+    ]
+
+
+def ops_bench(dev, n=24, seed=5, reps=20):
+    """Each drop-in call on one n-qubit complex64 state, out of place as the
+    reference's contract requires: time per call (events around `reps`
+    back-to-back calls on the current stream) and algorithmic GB/s."""
+    import torch
+    import unitair_amd as ua
+    from unitair_amd import simulation as sim
+    psi, other, u, ang, diag = _ops_inputs(n, seed, dev, torch.complex64)
+    out = {}
+    for name, fn, nbytes in _op_table(n, sim, ua.states, psi, other, u, ang, diag):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / reps * 1e3
+        out[name] = {"us_per_call": us, "algorithmic_bytes": nbytes,
+                     "algorithmic_GBps": nbytes / us / 1e3,
+                     "frac_of_hbm_peak": nbytes / us / 1e3 / HBM_PEAK_GBS}
+    return out
+
+
+def cpu_ops_baseline(n=24, seed=5, budget_s=10.0):
+    """The same calls through the torch CPU restatement of the reference
+    (one call each, in table order, until budget_s is spent)."""
+    import torch
+    from oracle import unitair_torch as T
+    psi, other, u, ang, diag = _ops_inputs(n, seed, "cpu", torch.complex64)
+
+    class _Sim:
+        apply_operator = staticmethod(lambda op, q, st: T.apply_operator(op, q, st))
+        act_first_qubits = staticmethod(
+            lambda op, st: T.apply_operator(op, list(range(op.shape[-1].bit_length() - 1)), st))
+        apply_phase = staticmethod(T.apply_phase)
+        apply_all_qubits = staticmethod(T.apply_all_qubits)
+
+    class _States:
+        norm_squared = staticmethod(T.norm_squared)
+        abs_squared = staticmethod(T.abs_squared)
+        inner_product = staticmethod(T.inner_product)
+        diag_expectation_value = staticmethod(T.diag_expectation_value)
+
+    out = {}
+    t_start = time.perf_counter()
+    for name, fn, nbytes in _op_table(n, _Sim, _States, psi, other, u, ang, diag):
+        t0 = time.perf_counter()
+        fn()
+        us = (time.perf_counter() - t0) * 1e6
+        out[name] = {"us_per_call": us, "algorithmic_GBps": nbytes / us / 1e3}
+        if time.perf_counter() - t_start > budget_s:
+            break
+    return out
+
+
+def measured_traffic():
+    """HBM-side bytes per launch of the 1-qubit dense kernels at 24 qubits,
+    from the FETCH_SIZE / WRITE_SIZE counter passes (scaled by a copy kernel
+    of known size) summarised in profiles/traffic_24q_circuit.md."""
+    path = ROOT / "profiles" / "traffic_24q_circuit.md"
+    try:
+        for line in path.read_text().splitlines():
+            if line.startswith("- traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def fidelity(circ, n, seed):
     import numpy as np
     import torch
@@ -254,6 +452,15 @@ def main():
     dev = torch.device("cuda:0")
     state = seeded_state(n, (), seed=args.seed + 10, dtype=torch.complex64, device=dev)
     runner = CircuitRunner(circ, dtype=torch.complex64, device=dev, in_place=True)
+    if args.copy_calibration_mib > 0:
+        numel = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(numel, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"copy calibration: {args.copy_calibration_mib} MiB read + written")
     log(f"timing {args.steps} steps (warmup {args.warmup}), in place")
     sec, per_kind = time_steps(runner, state, args.steps, args.warmup, with_events=True)
     value = ngates * amps / sec
@@ -269,27 +476,29 @@ def main():
         extra["out_of_place_ms_per_step"] = sec2 * 1e3
         log(f"out of place: {sec2 * 1e3:.3f} ms/step")
         del r2
-    log("timing graph replay")
-    r3 = CircuitRunner(circ, dtype=torch.complex64, device=dev, in_place=True)
-    sec3 = time_graph(r3, state, args.steps, args.warmup)
-    extra["graph_replay_amp_updates_per_s"] = ngates * amps / sec3
-    extra["graph_replay_ms_per_step"] = sec3 * 1e3
-    log(f"graph replay: {sec3 * 1e3:.3f} ms/step")
-    del r3
+    if not args.no_graph:
+        log("timing graph replay")
+        r3 = CircuitRunner(circ, dtype=torch.complex64, device=dev, in_place=True)
+        sec3 = time_graph(r3, state, args.steps, args.warmup)
+        extra["graph_replay_amp_updates_per_s"] = ngates * amps / sec3
+        extra["graph_replay_ms_per_step"] = sec3 * 1e3
+        log(f"graph replay: {sec3 * 1e3:.3f} ms/step")
+        del r3
 
-    log("timing fused passes")
-    from unitair_amd.fusion import FusedProgram
-    prog = FusedProgram(circ, dtype=torch.complex64, device=dev)
-    sec4, _ = time_steps(prog, state, args.steps, args.warmup, with_events=False)
-    sec5 = time_graph(prog, state, args.steps, args.warmup)
-    extra["fused"] = {"passes": prog.num_passes,
-                      "ms_per_step": sec4 * 1e3,
-                      "amp_updates_per_s": ngates * amps / sec4,
-                      "graph_ms_per_step": sec5 * 1e3,
-                      "graph_amp_updates_per_s": ngates * amps / sec5}
-    log(f"fused: {prog.num_passes} passes, {sec4 * 1e3:.3f} ms/step eager, "
-        f"{sec5 * 1e3:.3f} ms/step graph")
-    del prog
+    if not args.no_fused:
+        log("timing fused passes")
+        from unitair_amd.fusion import FusedProgram
+        prog = FusedProgram(circ, dtype=torch.complex64, device=dev)
+        sec4, _ = time_steps(prog, state, args.steps, args.warmup, with_events=False)
+        sec5 = time_graph(prog, state, args.steps, args.warmup)
+        extra["fused"] = {"passes": prog.num_passes,
+                          "ms_per_step": sec4 * 1e3,
+                          "amp_updates_per_s": ngates * amps / sec4,
+                          "graph_ms_per_step": sec5 * 1e3,
+                          "graph_amp_updates_per_s": ngates * amps / sec5}
+        log(f"fused: {prog.num_passes} passes, {sec4 * 1e3:.3f} ms/step eager, "
+            f"{sec5 * 1e3:.3f} ms/step graph")
+        del prog
 
     kernels = {k: {"launches_per_step": len(v) // args.steps,
                    "avg_us": 1e3 * float(np.mean(v))} for k, v in per_kind.items()}
@@ -298,7 +507,8 @@ def main():
     avg_s = kernels[dom]["avg_us"] * 1e-6
     achieved = dom_bytes / avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic() if n == 24 else None,
                 "kernel": "dense_fast_kernel<complex64,k=1>",
                 "bytes_per_launch": dom_bytes}
     # whole-step algorithmic bandwidth (the metric's "HBM GB/s")
@@ -325,16 +535,30 @@ def main():
             f"{sv['one_at_a_time']['p50_ms']:.1f} p99 {sv['one_at_a_time']['p99_ms']:.1f} ms, "
             f"{sv['one_at_a_time']['circuits_per_s']:.1f} circ/s")
 
+    if not args.no_fused_dense:
+        log("fused dense 4/5-qubit blocks, 28 qubits")
+        extra["fused_dense_blocks"] = fused_dense_bench(dev)
+
+    if not args.no_ops:
+        log(f"drop-in calls, {n} qubits")
+        extra["ops"] = ops_bench(dev, n=n)
+
+    # ---- CPU leg: the oracle as the timed baseline and as the checker of
+    # the GPU results (never as the thing measured on the GPU)
     if not args.no_fidelity:
         log("fidelity vs complex128 oracle")
         fe, mae = fidelity(circ, n, args.seed)
         extra["fidelity_error"] = fe
         extra["max_abs_error"] = mae
+        if not args.no_fused_dense:
+            extra["fused_dense_blocks"]["slice_check"] = fused_dense_slice_fidelity(dev)
 
     cpu = None
     if args.cpu_seconds > 0:
         log("cpu baseline")
         cpu = cpu_baseline(circ, n, args.seed, args.cpu_seconds)
+        if not args.no_ops:
+            cpu["ops"] = cpu_ops_baseline(n=n, budget_s=args.cpu_seconds)
 
     line = {
         "metric": "amplitude-updates/s (24q complex64 random circuit)",

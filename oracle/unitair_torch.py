@@ -8,7 +8,9 @@ It replays the reference's torch calls step for step:
                    einsum('ab...,b...->a...') (operations.py:322) ->
                    inverse permute().contiguous() -> view
   apply_phase      operations.py:41-42   exp(-1j*angles) * state
-  norm / <Z>       states/innerprod.py:29-59
+  apply_all_qubits operations.py:332-413 act on qubit 0, then (swap 0<->i,
+                   act on qubit 0) for i = 1..n-1, roll by -1, contiguous
+  reductions       states/innerprod.py:4-65
 """
 from __future__ import annotations
 
@@ -70,3 +72,32 @@ def norm_squared(state: torch.Tensor):
 def diag_expectation_value(diag: torch.Tensor, state: torch.Tensor):
     """innerprod.py:49-59."""
     return torch.sum((state.conj() * state).real * diag, dim=-1)
+
+
+def apply_all_qubits(op: torch.Tensor, state: torch.Tensor):
+    """operations.py:332-413 (swap_tensor :520-537, roll_qubits_tensor
+    :565-597, to_vector_layout states/conversions.py:48-92)."""
+    N = state.shape[-1]
+    n = N.bit_length() - 1
+    st = state.reshape(state.shape[:-1] + (2,) * n)
+    st = act_first_qubits_tensor(op, st, n, 1)
+    for i in range(1, n):
+        sbd = st.dim() - n
+        st = st.transpose(sbd, sbd + i)
+        st = act_first_qubits_tensor(op, st, n, 1)
+    sbd = st.dim() - n
+    steps = (-1) % n
+    if steps:
+        ident = list(range(sbd, sbd + n))
+        st = st.permute(list(range(sbd)) + ident[-steps:] + ident[:-steps])
+    return st.contiguous().reshape(st.shape[:sbd] + (N,))
+
+
+def abs_squared(state: torch.Tensor):
+    """innerprod.py:4-26."""
+    return (state.conj() * state).real
+
+
+def inner_product(a: torch.Tensor, b: torch.Tensor):
+    """innerprod.py:62-65."""
+    return torch.sum(a.conj() * b, dim=-1)

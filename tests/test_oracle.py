@@ -279,6 +279,34 @@ def test_reductions_textbook():
                           sum(abs(x) ** 2 * w for x, w in zip(a[r], d)))
 
 
+def test_torch_restatement_matches_pinned_numpy_oracle():
+    # the timed CPU baseline (oracle/unitair_torch.py) computes what the
+    # pinned restatement computes
+    import torch
+    from oracle import unitair_torch as T
+    n = 6
+    psi = rand_c((2, 2 ** n))
+    tp = torch.from_numpy(psi)
+    for qubits in ([0], [5], [4, 1], [2, 0, 5]):
+        op = rand_c((2 ** len(qubits),) * 2)
+        assert np.allclose(T.apply_operator(torch.from_numpy(op), qubits, tp).numpy(),
+                           U.apply_operator(op, qubits, psi), atol=1e-12)
+    u = rand_c((2, 2))
+    assert np.allclose(T.apply_all_qubits(torch.from_numpy(u), tp).numpy(),
+                       U.apply_all_qubits(u, psi), atol=1e-12)
+    ang = RNG.uniform(0, 2 * np.pi, 2 ** n)
+    assert np.allclose(T.apply_phase(torch.from_numpy(ang), tp).numpy(),
+                       U.apply_phase(ang, psi), atol=1e-12)
+    other = rand_c((2, 2 ** n))
+    d = RNG.standard_normal(2 ** n)
+    assert np.allclose(T.norm_squared(tp).numpy(), U.norm_squared(psi))
+    assert np.allclose(T.abs_squared(tp).numpy(), U.abs_squared(psi))
+    assert np.allclose(T.inner_product(tp, torch.from_numpy(other)).numpy(),
+                       U.inner_product(psi, other))
+    assert np.allclose(T.diag_expectation_value(torch.from_numpy(d), tp).numpy(),
+                       U.diag_expectation_value(d, psi))
+
+
 # ----------------------------------------------------------------- circuits --
 
 def test_circuit_runners_agree():
