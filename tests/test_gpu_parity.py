@@ -127,7 +127,8 @@ def test_act_first_qubits(device):
 @pytest.mark.parametrize("dt", [torch.complex64, torch.complex128, torch.float64])
 def test_apply_all_qubits(device, dt):
     rng = np.random.default_rng(21)
-    for n in (1, 2, 5, 9, 12):
+    # n >= 12 (complex) runs the tiled passes: 1, 2 and 3 passes below
+    for n in (1, 2, 5, 9, 12, 13, 17, 21):
         for ob, sb in [((), ()), ((), (3,)), ((3,), (3,)), ((4,), ())]:
             op = cast_np(rand(ob + (2, 2), dt, rng) / np.sqrt(2), dt)
             st = cast_np(rand(sb + (2 ** n,), dt, rng) / np.sqrt(2 ** n), dt)
