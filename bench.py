@@ -338,6 +338,8 @@ def _op_table(n, sim, states, psi, other, u, ang, diag):
          lambda: sim.apply_phase(ang, psi), 2 * A * s + A * 4),
         ("apply_all_qubits Haar 2x2",
          lambda: sim.apply_all_qubits(u[1], psi), 2 * A * s),
+        ("apply_to_qubits, 8 Haar 1q gates on 6 qubits (fused per qubit)",
+         lambda: sim.apply_to_qubits([u[1]] * 8, [0, 5, 11, 12, 17, 23, 0, 5], psi), 2 * A * s),
         ("norm_squared", lambda: states.norm_squared(psi), A * s),
         ("abs_squared", lambda: states.abs_squared(psi), A * s + A * 4),
         ("inner_product", lambda: states.inner_product(psi, other), 2 * A * s),
@@ -385,6 +387,7 @@ def cpu_ops_baseline(n=24, seed=5, budget_s=10.0):
             lambda op, st: T.apply_operator(op, list(range(op.shape[-1].bit_length() - 1)), st))
         apply_phase = staticmethod(T.apply_phase)
         apply_all_qubits = staticmethod(T.apply_all_qubits)
+        apply_to_qubits = staticmethod(T.apply_to_qubits)
 
     class _States:
         norm_squared = staticmethod(T.norm_squared)

@@ -126,6 +126,20 @@ int ua_apply_all_qubits(void* out, const void* state, const void* op, int dtype,
                         ua_stream_t stream);
 
 /*
+ * ua_apply_1q_layer -- replaces unitair.simulation.apply_to_qubits
+ * (unitair/simulation/operations.py:416-446) and apply_to_qubits_tensor
+ * (operations.py:449-503) after its per-qubit fusion
+ * (unitair/gates/matrix_algebra.py:6-51): one 2x2 operator on each of
+ * num_targets distinct qubits.  ops holds num_targets row-major 2x2 matrices
+ * per batch entry (target j at ops + 4 j); ops_batch_stride is 0 (shared) or
+ * 4 * num_targets.  With num_targets = 0 the state is copied.  Out of place.
+ */
+int ua_apply_1q_layer(void* out, const void* state, const void* ops, int dtype,
+                      int num_qubits, int num_targets, const int* qubits, int64_t batch,
+                      int64_t state_batch_stride, int64_t ops_batch_stride,
+                      ua_stream_t stream);
+
+/*
  * Reductions over the last dimension with float64 accumulation; replace
  * unitair/states/innerprod.py.  `length` elements per row, `batch` rows.
  * Each input row r starts at ptr + r*row_stride (0 = broadcast).  The result

@@ -136,6 +136,45 @@ def apply_all_qubits(op: np.ndarray, state: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(st).reshape(st.shape[:b] + (2 ** n,))
 
 
+def fuse_single_qubit_operators(qubits, operators):
+    """gates/matrix_algebra.py:6-51: gates on the same qubit are multiplied,
+    later gate on the left."""
+    fused = {}
+    for q, op in zip(qubits, operators):
+        fused[q] = np.matmul(op, fused[q]) if q in fused else op
+    return fused
+
+
+def apply_to_qubits(operators, qubits, state: np.ndarray) -> np.ndarray:
+    """simulation/operations.py:416-503: fuse per qubit; act on qubit 0 first;
+    for every other qubit swap its dim with dim 0 and act on dim 0, tracking
+    the permutation; finally permute the qubit dims back."""
+    n = count_qubits(state)
+    st = state.reshape(state.shape[:-1] + (2,) * n)
+    fused = fuse_single_qubit_operators(qubits, operators)
+    if 0 in fused:
+        st = act_first_qubits_tensor(fused.pop(0), st, n, 1)
+    perm = list(range(n))
+    for q, gate in fused.items():
+        perm[q], perm[0] = perm[0], q
+        b = st.ndim - n
+        st = np.swapaxes(st, b, b + q)                      # swap_tensor :520-537
+        st = act_first_qubits_tensor(gate, np.ascontiguousarray(st), n, 1)
+    inv = inverse_list_permutation(perm)
+    b = st.ndim - n
+    st = np.transpose(st, list(range(b)) + [b + p for p in inv])   # :626-654
+    return np.ascontiguousarray(st).reshape(st.shape[:b] + (2 ** n,))
+
+
+def act_last_qubit(op: np.ndarray, state: np.ndarray) -> np.ndarray:
+    """simulation/operations.py:189-233: einsum('ab, ...b -> ...a') over the
+    last qubit's tensor dim."""
+    n = count_qubits(state)
+    st = state.reshape(state.shape[:-1] + (2 ** (n - 1), 2))
+    out = np.einsum("ab,...b->...a", op, st)
+    return out.reshape(state.shape)
+
+
 # ---------------------------------------------------------------- reductions
 
 def abs_squared(state):

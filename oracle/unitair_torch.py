@@ -10,6 +10,7 @@ It replays the reference's torch calls step for step:
   apply_phase      operations.py:41-42   exp(-1j*angles) * state
   apply_all_qubits operations.py:332-413 act on qubit 0, then (swap 0<->i,
                    act on qubit 0) for i = 1..n-1, roll by -1, contiguous
+  apply_to_qubits  operations.py:416-503 (fuse per qubit, swap, act, permute)
   reductions       states/innerprod.py:4-65
 """
 from __future__ import annotations
@@ -90,6 +91,29 @@ def apply_all_qubits(op: torch.Tensor, state: torch.Tensor):
     if steps:
         ident = list(range(sbd, sbd + n))
         st = st.permute(list(range(sbd)) + ident[-steps:] + ident[:-steps])
+    return st.contiguous().reshape(st.shape[:sbd] + (N,))
+
+
+def apply_to_qubits(operators, qubits, state: torch.Tensor):
+    """operations.py:416-503 with fuse_single_qubit_operators
+    (gates/matrix_algebra.py:6-51)."""
+    N = state.shape[-1]
+    n = N.bit_length() - 1
+    fused = {}
+    for q, op in zip(qubits, operators):
+        fused[q] = torch.matmul(op, fused[q]) if q in fused else op
+    st = state.reshape(state.shape[:-1] + (2,) * n)
+    if 0 in fused:
+        st = act_first_qubits_tensor(fused.pop(0), st, n, 1)
+    perm = list(range(n))
+    for q, gate in fused.items():
+        perm[q], perm[0] = perm[0], q
+        sbd = st.dim() - n
+        st = st.transpose(sbd, sbd + q)
+        st = act_first_qubits_tensor(gate, st, n, 1)
+    _, inv = _to_front(n, perm)
+    sbd = st.dim() - n
+    st = st.permute(list(range(sbd)) + [sbd + p for p in inv])
     return st.contiguous().reshape(st.shape[:sbd] + (N,))
 
 

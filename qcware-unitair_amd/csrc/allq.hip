@@ -1,20 +1,24 @@
-// Same 2x2 operator on every qubit: ua_apply_all_qubits
-// (unitair/simulation/operations.py:332-413).
+// 1-qubit gate layers: ua_apply_all_qubits (the same 2x2 operator on every
+// qubit, unitair/simulation/operations.py:332-413) and ua_apply_1q_layer (one
+// operator per listed qubit, the fused form of apply_to_qubits,
+// operations.py:416-503).
 //
 // The reference loops over qubits with swap (a transposed view) + einsum and
-// finishes with a roll and a contiguous copy: n passes over the state plus
-// the copy.  Here the 1-qubit gates, which all commute, are grouped into
-// tiled passes.  A workgroup loads a 2^12-amplitude tile into LDS; the tile
-// spans 12 index bits ("local bits").  In three stages every thread holds a
-// cube of 16 amplitudes over 4 of the local bits and applies U to each of
-// those bits that the pass owns, in registers.
-//   pass 0: local bits 0..11, U applied to bits 0..a0-1;
-//   pass p: local bits 0..3 (not applied; they keep every access a 128 B
-//           run) plus 8 further bits, applied.
-// 24 qubits take 3 passes instead of 24.  The first pass reads the input and
-// writes `out`; later passes run in place on `out`.  Complex states with at
-// least 12 qubits use the tiled passes; other cases apply U qubit by qubit
-// with the dense kernels.
+// finishes with a permute/roll and a contiguous copy: one pass over the state
+// per qubit plus the copy.  Gates on distinct qubits commute, so here they are
+// grouped into tiled passes.  A workgroup loads a 2^12-amplitude tile into
+// LDS; the tile spans 12 index bits ("local bits").  In up to three stages
+// every thread holds a cube of 16 amplitudes over 4 of the local bits and
+// applies each target bit's 2x2 matrix in registers.
+//   low pass:  local bits 0..11, gates on target bits among them;
+//   high pass: local bits 0..3 (no gates there; they keep every access a
+//              128 B run) plus 8 further bits carrying up to 8 gates.
+// Low targets on bits 4..11 move into spare slots of the high passes, so a
+// gate set takes the fewest passes and no pass carries more than 12 gates.
+// 24 qubits of apply_all_qubits take 3 passes instead of 24.  The first pass
+// reads the input and writes `out`; later passes run in place on `out`.
+// Complex states with at least 12 qubits use the tiled passes; other cases
+// apply the gates qubit by qubit with the dense kernels.
 #include "ua_common.h"
 
 namespace ua {
@@ -70,8 +74,10 @@ struct AllqPass {
   int n;
   int log2_tiles;            // n - 12
   int local[kTileBits];      // ascending state bits of the tile
+  int64_t target_stride;     // elements between consecutive targets' matrices (0: shared)
   int stage_bits[3][4];      // tile bits of each stage's cube
-  int stage_apply[3];        // 4-bit mask: cube bits that get U
+  int stage_apply[3];        // 4-bit mask: cube bits that get a gate
+  int stage_slot[3][4];      // target index of each cube bit's gate
   int num_stages;
 };
 
@@ -110,8 +116,7 @@ __global__ __launch_bounds__(kThreads) void allq_pass_kernel(T* out, const T* in
     tile[lds_pos(e + 1)] = v.v[1];
   }
 
-  const T* u = op + b * a.op_bstride;
-  const T u00 = u[0], u01 = u[1], u10 = u[2], u11 = u[3];
+  const T* ops_b = op + b * a.op_bstride;
 
   for (int st = 0; st < a.num_stages; ++st) {
     __syncthreads();
@@ -127,10 +132,16 @@ __global__ __launch_bounds__(kThreads) void allq_pass_kernel(T* out, const T* in
       c[j] = tile[lds_pos(e)];
     }
     const int act = a.stage_apply[st];
-    if (act & 1) cube_1q<0>(c, u00, u01, u10, u11);
-    if (act & 2) cube_1q<1>(c, u00, u01, u10, u11);
-    if (act & 4) cube_1q<2>(c, u00, u01, u10, u11);
-    if (act & 8) cube_1q<3>(c, u00, u01, u10, u11);
+#define UA_LAYER_BIT(K)                                                       \
+    if (act & (1 << K)) {                                                     \
+      const T* u = ops_b + a.stage_slot[st][K] * a.target_stride;             \
+      cube_1q<K>(c, u[0], u[1], u[2], u[3]);                                  \
+    }
+    UA_LAYER_BIT(0)
+    UA_LAYER_BIT(1)
+    UA_LAYER_BIT(2)
+    UA_LAYER_BIT(3)
+#undef UA_LAYER_BIT
 #pragma unroll
     for (int j = 0; j < kCube; ++j) {
       const int e = e0 | ((j & 1) ? m0 : 0) | ((j & 2) ? m1 : 0) | ((j & 4) ? m2 : 0) |
@@ -150,37 +161,70 @@ __global__ __launch_bounds__(kThreads) void allq_pass_kernel(T* out, const T* in
   }
 }
 
-// Split the n state bits into tiled passes (see the file comment).  The
-// fewest passes are P = 1 + ceil((n - 12) / 8); the gates are spread so that
-// pass 0 applies bits 0..a0-1 (a0 = n - 8 (P - 1), between 4 and 12) and every
-// later pass applies 8 bits.  Spreading them keeps each pass below its memory
-// time (12 gates in one pass are VALU-bound without packed FP32).  Returns the
-// number of passes written to `passes` (at most 8).
-int plan_allq(int n, AllqPass* passes) {
-  const int P = 1 + (n - kTileBits + 7) / 8;
-  const int a0 = n - 8 * (P - 1);
+// Plan the passes for the gates on the state bits in `active` (slot[bit] =
+// the bit's target index).  High targets (bits >= 12) go 8 to a pass; low
+// targets on bits 4..11 fill the high passes' spare slots, highest first; the
+// rest of the low targets share one pass over bits 0..11.  With no target at
+// all, one pass without stages copies the input.  Returns the number of
+// passes (at most 8).
+int plan_layer(int n, uint64_t active, const int* slot, AllqPass* passes) {
+  uint64_t low = active & 0xFFFull, high = active & ~0xFFFull;
+  const int H = __builtin_popcountll(high);
+  int spare = 8 * ((H + 7) / 8) - H;
+  uint64_t movable = low & 0xFF0ull;
+  while (spare > 0 && movable) {
+    const int bit = 63 - __builtin_clzll(movable);
+    movable &= ~(uint64_t(1) << bit);
+    low &= ~(uint64_t(1) << bit);
+    high |= uint64_t(1) << bit;
+    --spare;
+  }
   int np = 0;
-  {
+  if (low || !high) {
     AllqPass& p = passes[np++];
     for (int i = 0; i < kTileBits; ++i) p.local[i] = i;
     p.num_stages = 0;
     for (int s = 0; s < 3; ++s) {
       int mask = 0;
       for (int k = 0; k < 4; ++k)
-        if (4 * s + k < a0) mask |= 1 << k;
+        if ((low >> (4 * s + k)) & 1) mask |= 1 << k;
       if (!mask) continue;
-      for (int k = 0; k < 4; ++k) p.stage_bits[p.num_stages][k] = 4 * s + k;
+      for (int k = 0; k < 4; ++k) {
+        p.stage_bits[p.num_stages][k] = 4 * s + k;
+        p.stage_slot[p.num_stages][k] = (mask >> k) & 1 ? slot[4 * s + k] : 0;
+      }
       p.stage_apply[p.num_stages++] = mask;
     }
   }
-  for (int hi0 = a0; hi0 < n; hi0 += 8) {
+  while (high) {
     AllqPass& p = passes[np++];
+    uint64_t chunk = 0;
+    for (int c = 0; c < 8 && high; ++c) {
+      const uint64_t lowest = high & (~high + 1);
+      chunk |= lowest;
+      high &= ~lowest;
+    }
+    // complete the eight high local bits with bits >= 4 that carry no gate here
+    uint64_t local_hi = chunk;
+    for (int bit = 4; __builtin_popcountll(local_hi) < 8; ++bit)
+      local_hi |= uint64_t(1) << bit;
     for (int i = 0; i < 4; ++i) p.local[i] = i;
-    for (int i = 0; i < 8; ++i) p.local[4 + i] = hi0 + i;
-    p.num_stages = 2;
+    for (int i = 4; i < kTileBits; ++i) {
+      p.local[i] = __builtin_ctzll(local_hi);
+      local_hi &= local_hi - 1;
+    }
+    p.num_stages = 0;
     for (int s = 0; s < 2; ++s) {
-      for (int k = 0; k < 4; ++k) p.stage_bits[s][k] = 4 + 4 * s + k;
-      p.stage_apply[s] = 0xF;
+      int mask = 0;
+      for (int k = 0; k < 4; ++k)
+        if ((chunk >> p.local[4 + 4 * s + k]) & 1) mask |= 1 << k;
+      if (!mask) continue;
+      for (int k = 0; k < 4; ++k) {
+        const int t = 4 + 4 * s + k;
+        p.stage_bits[p.num_stages][k] = t;
+        p.stage_slot[p.num_stages][k] = (mask >> k) & 1 ? slot[p.local[t]] : 0;
+      }
+      p.stage_apply[p.num_stages++] = mask;
     }
   }
   return np;
@@ -189,36 +233,50 @@ int plan_allq(int n, AllqPass* passes) {
 }  // namespace
 }  // namespace ua
 
-extern "C" int ua_apply_all_qubits(void* out, const void* state, const void* op, int dtype,
-                                   int num_qubits, int64_t batch,
-                                   int64_t state_batch_stride, int64_t op_batch_stride,
-                                   ua_stream_t stream) {
-  using namespace ua;
-  UA_CHECK_ARG(num_qubits >= 1 && num_qubits <= 62, "apply_all_qubits needs 1..62 qubits");
-  UA_CHECK_ARG(out != state, "apply_all_qubits is out of place");
+namespace ua {
+namespace {
+
+// Gates on distinct qubits: the matrix of target j for batch entry b is at
+// op + b * op_bstride + j * target_stride.
+int layer_impl(void* out, const void* state, const void* op, int dtype, int n,
+               int num_targets, const int* qubits, int64_t batch, int64_t state_bstride,
+               int64_t op_bstride, int64_t target_stride, hipStream_t s) {
+  UA_CHECK_ARG(n >= 1 && n <= 62, "1-qubit layers need 1..62 qubits (got %d)", n);
+  UA_CHECK_ARG(out != state, "1-qubit layers are out of place");
   UA_CHECK_ARG(batch >= 0, "negative batch");
-  UA_CHECK_ARG(op_batch_stride == 0 || op_batch_stride == 4,
-               "op_batch_stride must be 0 or 4");
+  UA_CHECK_ARG(num_targets >= 0 && num_targets <= n, "num_targets=%d out of range",
+               num_targets);
+  UA_CHECK_ARG(op_bstride >= 0 && target_stride >= 0, "negative operator stride");
   if (batch == 0) return UA_OK;
-  UA_CHECK_ARG(out && state && op, "NULL pointer");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const int64_t N = int64_t(1) << num_qubits;
+  UA_CHECK_ARG(out && state && (op || num_targets == 0), "NULL pointer");
+  UA_CHECK_ARG(num_targets == 0 || qubits, "NULL qubits");
+  int slot[64];
+  uint64_t active = 0;
+  for (int j = 0; j < num_targets; ++j) {
+    const int q = qubits[j];
+    UA_CHECK_ARG(q >= 0 && q < n, "qubit %d out of range for %d qubits", q, n);
+    const int bit = n - 1 - q;
+    UA_CHECK_ARG(!((active >> bit) & 1), "qubit %d listed twice", q);
+    active |= uint64_t(1) << bit;
+    slot[bit] = j;
+  }
+  const int64_t N = int64_t(1) << n;
   const bool complex = dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128;
   const size_t es = complex ? elem_size(dtype) : 0;
-  const bool aligned = complex &&
-                       reinterpret_cast<uintptr_t>(out) % (2 * es) == 0 &&
+  const bool aligned = complex && reinterpret_cast<uintptr_t>(out) % (2 * es) == 0 &&
                        reinterpret_cast<uintptr_t>(state) % (2 * es) == 0 &&
-                       state_batch_stride % 2 == 0;
-  if (num_qubits >= kTileBits && aligned) {
+                       state_bstride % 2 == 0;
+  if (n >= kTileBits && aligned) {
     AllqPass passes[8];
-    const int np = plan_allq(num_qubits, passes);
+    const int np = plan_layer(n, active, slot, passes);
     for (int i = 0; i < np; ++i) {
       AllqPass& p = passes[i];
-      p.n = num_qubits;
-      p.log2_tiles = num_qubits - kTileBits;
+      p.n = n;
+      p.log2_tiles = n - kTileBits;
       p.total_tiles = batch << p.log2_tiles;
-      p.in_bstride = (i == 0) ? state_batch_stride : N;
-      p.op_bstride = op_batch_stride;
+      p.in_bstride = (i == 0) ? state_bstride : N;
+      p.op_bstride = op_bstride;
+      p.target_stride = target_stride;
       UA_CHECK_ARG(p.total_tiles <= 0x7fffffffLL, "too many tiles");
       const void* src = (i == 0) ? state : out;
       if (dtype == UA_COMPLEX64)
@@ -234,13 +292,58 @@ extern "C" int ua_apply_all_qubits(void* out, const void* state, const void* op,
     }
     return UA_OK;
   }
-  for (int q = 0; q < num_qubits; ++q) {
-    const int qubit[1] = {q};
-    const void* src = (q == 0) ? state : out;
-    const int64_t sstride = (q == 0) ? state_batch_stride : N;
-    const int rc = apply_operator_impl(out, src, op, dtype, num_qubits, 1, qubit, batch,
-                                       sstride, op_batch_stride, s);
+  UA_CHECK_ARG(dtype >= UA_FLOAT32 && dtype <= UA_COMPLEX128, "bad dtype %d", dtype);
+  if (num_targets == 0) {
+    const size_t bytes = size_t(N) * elem_size(dtype);
+    for (int64_t b = 0; b < batch; ++b) {
+      const hipError_t e = hipMemcpyAsync(
+          static_cast<char*>(out) + b * bytes,
+          static_cast<const char*>(state) + b * state_bstride * elem_size(dtype), bytes,
+          hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) {
+        set_error("hipMemcpyAsync: %s", hipGetErrorString(e));
+        return UA_ERR_HIP;
+      }
+    }
+    return UA_OK;
+  }
+  const size_t mes = elem_size(dtype);
+  for (int j = 0; j < num_targets; ++j) {
+    const int qubit[1] = {qubits[j]};
+    const void* src = (j == 0) ? state : out;
+    const int64_t sstride = (j == 0) ? state_bstride : N;
+    const void* u = static_cast<const char*>(op) + size_t(j * target_stride) * mes;
+    const int rc = apply_operator_impl(out, src, u, dtype, n, 1, qubit, batch, sstride,
+                                       op_bstride, s);
     if (rc) return rc;
   }
   return UA_OK;
+}
+
+}  // namespace
+}  // namespace ua
+
+extern "C" int ua_apply_all_qubits(void* out, const void* state, const void* op, int dtype,
+                                   int num_qubits, int64_t batch,
+                                   int64_t state_batch_stride, int64_t op_batch_stride,
+                                   ua_stream_t stream) {
+  using namespace ua;
+  UA_CHECK_ARG(num_qubits >= 1 && num_qubits <= 62, "apply_all_qubits needs 1..62 qubits");
+  UA_CHECK_ARG(op_batch_stride == 0 || op_batch_stride == 4,
+               "op_batch_stride must be 0 or 4");
+  int qubits[62];
+  for (int q = 0; q < num_qubits; ++q) qubits[q] = q;
+  return layer_impl(out, state, op, dtype, num_qubits, num_qubits, qubits, batch,
+                    state_batch_stride, op_batch_stride, 0, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int ua_apply_1q_layer(void* out, const void* state, const void* ops, int dtype,
+                                 int num_qubits, int num_targets, const int* qubits,
+                                 int64_t batch, int64_t state_batch_stride,
+                                 int64_t ops_batch_stride, ua_stream_t stream) {
+  using namespace ua;
+  UA_CHECK_ARG(ops_batch_stride == 0 || ops_batch_stride == 4 * int64_t(num_targets),
+               "ops_batch_stride must be 0 or 4 * num_targets");
+  return layer_impl(out, state, ops, dtype, num_qubits, num_targets, qubits, batch,
+                    state_batch_stride, ops_batch_stride, 4, static_cast<hipStream_t>(stream));
 }

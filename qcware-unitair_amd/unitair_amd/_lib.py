@@ -49,6 +49,7 @@ SIGNATURES = {
     "ua_apply_operator": (_i32, [_p, _p, _p, _i32, _i32, _i32, _pi32, _i64, _i64, _i64, _p]),
     "ua_apply_phase": (_i32, [_p, _i32, _p, _i32, _p, _i32, _i32, _pi64, _pi64, _pi64, _p]),
     "ua_apply_all_qubits": (_i32, [_p, _p, _p, _i32, _i32, _i64, _i64, _i64, _p]),
+    "ua_apply_1q_layer": (_i32, [_p, _p, _p, _i32, _i32, _i32, _pi32, _i64, _i64, _i64, _p]),
     "ua_reduce_workspace_bytes": (_i64, [_i64, _i64]),
     "ua_norm_squared": (_i32, [_p, _i32, _p, _i32, _i64, _i64, _i64, _p, _p]),
     "ua_inner_product": (_i32, [_p, _i32, _p, _i64, _p, _i64, _i32, _i64, _i64, _p, _p]),

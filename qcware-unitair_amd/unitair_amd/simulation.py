@@ -21,7 +21,8 @@ from . import _lib
 from ._device import on_device, prepare, require_gpu, same_device
 from .states import count_qubits, count_qubits_gate_matrix
 
-__all__ = ["apply_operator", "act_first_qubits", "apply_phase", "apply_all_qubits"]
+__all__ = ["apply_operator", "act_first_qubits", "apply_phase", "apply_all_qubits",
+           "apply_to_qubits", "act_last_qubit"]
 
 _SUPPORTED = tuple(_lib.DTYPE_CODES)
 
@@ -317,6 +318,91 @@ def apply_all_qubits(operator: torch.Tensor, state: torch.Tensor) -> torch.Tenso
             n, batch, s_stride, op_bstride, _lib.stream_handle(state.device))
     _lib.check(rc, "apply_all_qubits")
     return out
+
+
+def apply_to_qubits(operators: Iterable[torch.Tensor], qubits: Iterable[int],
+                    state: torch.Tensor) -> torch.Tensor:
+    """Apply 1-qubit gates to the listed qubits (operations.py:416-503).
+
+    Gates on the same qubit are multiplied first, in order
+    (fuse_single_qubit_operators, unitair/gates/matrix_algebra.py:6-51); the
+    fused gates, which act on distinct qubits and commute, then run as one
+    1-qubit layer in tiled passes (a few passes over the state instead of one
+    per qubit).  operators: (*batch_dims, 2, 2) each, all with the same batch
+    dims; zip(operators, qubits) pairs them as in the reference.  Batched
+    operators with the state's batch dims (or with an unbatched state) pair up
+    entry by entry; other broadcasts apply the fused gates one by one with the
+    reference's apply_operator rules.
+    """
+    from .gates.matrix_algebra import fuse_single_qubit_operators
+    n = count_qubits(state)
+    fused = fuse_single_qubit_operators(qubits, operators)
+    for q in fused:
+        if not (0 <= int(q) < n):
+            raise ValueError(f"Expected index in {{-num_qubits, ..., num_qubits - 1}}.\n"
+                             f"Num_qubits: {n}, index: {q}.")
+    qs = [int(q) for q in fused]
+    ops = list(fused.values())
+    for op in ops:
+        if count_qubits_gate_matrix(op) != 1:
+            raise ValueError(f"Expected operator on 1 qubit, found a "
+                             f"{count_qubits_gate_matrix(op)} qubit operator.")
+        _check_operator(op, state, "apply_to_qubits")
+    obatch = tuple(ops[0].shape[:-2]) if ops else ()
+    if any(tuple(op.shape[:-2]) != obatch for op in ops):
+        raise RuntimeError("apply_to_qubits: all operators must have the same batch dims")
+    sbatch = tuple(state.shape[:-1])
+    pairable = obatch == () or obatch == sbatch or sbatch == ()
+    if ops and (not pairable or _needs_grad(state, *ops)):
+        x = state
+        for q, op in zip(qs, ops):
+            x = apply_operator(op, [q], x)
+        return x
+    require_gpu(state, "apply_to_qubits")
+    if state.dtype not in _SUPPORTED:
+        raise TypeError(f"apply_to_qubits: dtype {state.dtype} is not supported")
+    state = prepare(state)
+    N = 2 ** n
+    k = len(qs)
+    if obatch and not sbatch:
+        out_batch, s_flat, s_stride = obatch, state.reshape(N).contiguous(), 0
+    else:
+        out_batch = sbatch
+        s_flat, s_stride = _flat_batch(state, N)
+    # (k, *obatch, 2, 2) -> (*obatch, k, 2, 2): target j of entry b at 4 (b k + j)
+    mats = (torch.stack([prepare(op) for op in ops], dim=-3).contiguous() if ops
+            else torch.empty(0, dtype=state.dtype, device=state.device))
+    batch = math.prod(out_batch)
+    out = torch.empty(tuple(out_batch) + (N,), dtype=state.dtype, device=state.device)
+    if batch == 0:
+        return out
+    lib = _lib.load()
+    with on_device(state.device):
+        rc = lib.ua_apply_1q_layer(
+            _lib.ptr(out), _lib.ptr(s_flat), _lib.ptr(mats), _lib.DTYPE_CODES[state.dtype], n,
+            k, _lib.int_array(qs), batch, s_stride, 4 * k if obatch else 0,
+            _lib.stream_handle(state.device))
+    _lib.check(rc, "apply_to_qubits")
+    return out
+
+
+def act_last_qubit(single_qubit_operator: torch.Tensor, state: torch.Tensor) -> torch.Tensor:
+    """Apply a 2x2 operator to the last qubit (operations.py:189-213); like the
+    reference, warns that the function is outdated.  The operator has no batch
+    dims (the reference contracts it with einsum('ab, ...b -> ...a'))."""
+    import warnings
+    warnings.warn(
+        'act_last_qubit and act_last_qubit_tensor are outdated. These\n'
+        'functions will be removed from Unitair in a later release or may\n'
+        'be revised to meet our current standards.\n'
+        'Please consider using apply_operator instead.')
+    n = count_qubits(state)
+    if single_qubit_operator.dim() != 2:
+        raise RuntimeError("einsum(): the number of subscripts in the equation (2) does not "
+                           "match the number of dimensions "
+                           f"({single_qubit_operator.dim()}) for operand 0 and no ellipsis "
+                           "was given")
+    return apply_operator(single_qubit_operator, [n - 1], state)
 
 
 # ------------------------------------------------------------------- phase --

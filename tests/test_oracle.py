@@ -229,6 +229,55 @@ def test_apply_all_qubits_batched_operator():
         assert np.allclose(out[i], U.apply_all_qubits(ops[i], psi[i]))
 
 
+def test_apply_to_qubits_equals_apply_all_qubits():
+    # reference tests/test_unitary.py:38-52 (exact equality there)
+    for n in (1, 7):
+        u = G.exp_x(0.3)
+        psi = rand_c(2 ** n)
+        assert np.allclose(U.apply_to_qubits([u] * n, range(n), psi),
+                           U.apply_all_qubits(u, psi), atol=1e-14)
+
+
+def test_apply_to_qubits_is_the_gate_sequence():
+    # fusion (later gate on the left) + permutation bookkeeping against the
+    # closed form of one apply_operator per gate, repeated qubits included
+    n = 6
+    qs = [4, 0, 4, 2, 5, 0, 1]
+    ops = [rand_unitary(2) for _ in qs]
+    psi = rand_c((3, 2 ** n))
+    seq = psi
+    for q, op in zip(qs, ops):
+        seq = closed_form.apply_dense(op, [q], seq)
+    assert np.allclose(U.apply_to_qubits(ops, qs, psi), seq, atol=1e-12)
+    # batched operators, one per state
+    bops = [rand_c((3, 2, 2)) for _ in qs]
+    out = U.apply_to_qubits(bops, qs, psi)
+    for r in range(3):
+        assert np.allclose(out[r], U.apply_to_qubits([o[r] for o in bops], qs, psi[r]))
+
+
+def test_fuse_single_qubit_operators_order():
+    # the docstring example of gates/matrix_algebra.py:16-35
+    A, B, C, D = (rand_c((2, 2)) for _ in range(4))
+    f = U.fuse_single_qubit_operators([2, 4, 2, 7], [A, B, C, D])
+    assert sorted(f) == [2, 4, 7]
+    assert np.allclose(f[2], C @ A) and f[4] is B and f[7] is D
+
+
+def test_act_last_qubit_equals_apply_to_last():
+    # reference tests/test_unitary.py:54-92
+    for n in range(1, 6):
+        op = rand_c((2, 2))
+        psi = rand_c(2 ** n)
+        assert np.allclose(U.act_last_qubit(op, psi), U.apply_to_qubits([op], [n - 1], psi))
+        assert np.allclose(U.act_last_qubit(op, psi), closed_form.apply_dense(op, [n - 1], psi))
+    op = rand_c((2, 2))
+    batch = rand_c((4, 3, 2, 2, 2, 2, 2))
+    out = U.act_last_qubit(op, batch)
+    assert np.allclose(out[2, 2], U.act_last_qubit(op, batch[2, 2]))
+    assert np.allclose(out[0, 1], U.act_last_qubit(op, batch[0, 1]))
+
+
 # ------------------------------------------------------------------- phase --
 
 def basic_phase(angles, state):
@@ -294,6 +343,10 @@ def test_torch_restatement_matches_pinned_numpy_oracle():
     u = rand_c((2, 2))
     assert np.allclose(T.apply_all_qubits(torch.from_numpy(u), tp).numpy(),
                        U.apply_all_qubits(u, psi), atol=1e-12)
+    qs = [3, 0, 3, 5]
+    ops = [rand_c((2, 2)) for _ in qs]
+    assert np.allclose(T.apply_to_qubits([torch.from_numpy(o) for o in ops], qs, tp).numpy(),
+                       U.apply_to_qubits(ops, qs, psi), atol=1e-12)
     ang = RNG.uniform(0, 2 * np.pi, 2 ** n)
     assert np.allclose(T.apply_phase(torch.from_numpy(ang), tp).numpy(),
                        U.apply_phase(ang, psi), atol=1e-12)
