@@ -322,7 +322,10 @@ def _ops_inputs(n, seed, device, dtype):
 
 def _op_table(n, sim, states, psi, other, u, ang, diag):
     """(name, callable, algorithmic bytes) for each drop-in call, complex64."""
+    import torch
     A, s = 2 ** n, 8
+    rev = list(range(n))[::-1]
+    ring = torch.tensor([[q, (q + 1) % n] for q in range(n)], device=psi.device)
     return [
         ("apply_operator 1q, qubit 0 (stride 2^(n-1))",
          lambda: sim.apply_operator(u[1], [0], psi), 2 * A * s),
@@ -339,7 +342,11 @@ def _op_table(n, sim, states, psi, other, u, ang, diag):
         ("apply_all_qubits Haar 2x2",
          lambda: sim.apply_all_qubits(u[1], psi), 2 * A * s),
         ("apply_to_qubits, 8 Haar 1q gates on 6 qubits (fused per qubit)",
-         lambda: sim.apply_to_qubits([u[1]] * 8, [0, 5, 11, 12, 17, 23, 0, 5], psi), 2 * A * s),
+         lambda: sim.apply_to_qubits([u[1]] * 8, [0, 5, 11, 12, 17, n - 1, 0, 5], psi),
+         2 * A * s),
+        ("swap qubits (0, n-1)", lambda: sim.swap(psi, (0, n - 1)), 2 * A * s),
+        ("permute_qubits, reversed order", lambda: sim.permute_qubits(rev, psi), 2 * A * s),
+        ("multi_cz, ring of n pairs", lambda: sim.multi_cz(ring, psi), 2 * A * s),
         ("norm_squared", lambda: states.norm_squared(psi), A * s),
         ("abs_squared", lambda: states.abs_squared(psi), A * s + A * 4),
         ("inner_product", lambda: states.inner_product(psi, other), 2 * A * s),
@@ -388,6 +395,9 @@ def cpu_ops_baseline(n=24, seed=5, budget_s=10.0):
         apply_phase = staticmethod(T.apply_phase)
         apply_all_qubits = staticmethod(T.apply_all_qubits)
         apply_to_qubits = staticmethod(T.apply_to_qubits)
+        swap = staticmethod(T.swap)
+        permute_qubits = staticmethod(T.permute_qubits)
+        multi_cz = staticmethod(T.multi_cz)
 
     class _States:
         norm_squared = staticmethod(T.norm_squared)

@@ -126,6 +126,39 @@ int ua_apply_all_qubits(void* out, const void* state, const void* op, int dtype,
                         ua_stream_t stream);
 
 /*
+ * ua_permute_qubits -- replaces unitair.simulation.permute_qubits
+ * (unitair/simulation/operations.py:600-623, permute_qubits_tensor :626-654),
+ * and with the matching permutations swap (:506-537) and roll_qubits
+ * (:540-597): output qubit q is input qubit permutation[q] (torch.permute on
+ * the qubit dims of the tensor layout).  Out of place; any dtype.
+ */
+int ua_permute_qubits(void* out, const void* state, int dtype, int num_qubits,
+                      const int* permutation, int64_t batch, int64_t state_batch_stride,
+                      ua_stream_t stream);
+
+/*
+ * ua_apply_sign_gates -- replaces unitair.simulation.multi_cz
+ * (operations.py:657-748) and multi_controlled_z (:751-783): out = s * state
+ * with s(i) = (-1)^(number of pairs whose two index bits are set in i),
+ * times -1 where every bit of all_mask is set when use_all is nonzero.
+ * pair_bits holds num_pairs (bit, bit) pairs of INDEX bits (qubit q is bit
+ * num_qubits-1-q).  Out of place; any dtype.
+ */
+int ua_apply_sign_gates(void* out, const void* state, int dtype, int num_qubits,
+                        int num_pairs, const int* pair_bits, int use_all, uint64_t all_mask,
+                        int64_t batch, int64_t state_batch_stride, ua_stream_t stream);
+
+/*
+ * ua_apply_controlled_flip -- replaces unitair.simulation.multi_controlled_x
+ * (operations.py:786-806, H . multi_controlled_z . H on the target): out[i] =
+ * state[i ^ 2^target_bit] where every bit of control_mask is set in i, else
+ * state[i].  Exact (no Hadamard round trip).  Out of place; any dtype.
+ */
+int ua_apply_controlled_flip(void* out, const void* state, int dtype, int num_qubits,
+                             uint64_t control_mask, int target_bit, int64_t batch,
+                             int64_t state_batch_stride, ua_stream_t stream);
+
+/*
  * ua_apply_1q_layer -- replaces unitair.simulation.apply_to_qubits
  * (unitair/simulation/operations.py:416-446) and apply_to_qubits_tensor
  * (operations.py:449-503) after its per-qubit fusion

@@ -175,6 +175,68 @@ def act_last_qubit(op: np.ndarray, state: np.ndarray) -> np.ndarray:
     return out.reshape(state.shape)
 
 
+def permute_qubits(permutation, state: np.ndarray) -> np.ndarray:
+    """simulation/operations.py:600-654: permute the qubit dims of the tensor
+    layout (output dim q = input dim permutation[q])."""
+    n = count_qubits(state)
+    st = state.reshape(state.shape[:-1] + (2,) * n)
+    b = st.ndim - n
+    st = np.transpose(st, list(range(b)) + [b + p for p in permutation])
+    return np.ascontiguousarray(st).reshape(state.shape)
+
+
+def swap(state: np.ndarray, qubit_pair) -> np.ndarray:
+    """simulation/operations.py:506-537: transpose of two qubit dims."""
+    n = count_qubits(state)
+    st = state.reshape(state.shape[:-1] + (2,) * n)
+    b = st.ndim - n
+    q1, q2 = (q + b if q >= 0 else q for q in qubit_pair)
+    return np.ascontiguousarray(np.swapaxes(st, q1, q2)).reshape(state.shape)
+
+
+def roll_qubits(state: np.ndarray, num_steps: int = 1) -> np.ndarray:
+    """simulation/operations.py:540-597."""
+    n = count_qubits(state)
+    k = num_steps % n
+    ident = list(range(n))
+    return permute_qubits(ident[-k:] + ident[:-k] if k else ident, state)
+
+
+def multi_cz(qubit_pairs: np.ndarray, state: np.ndarray) -> np.ndarray:
+    """simulation/operations.py:657-748: phase = prod over pairs of
+    (1 - 2 [x has both bits 2^(n-1-c), 2^(n-1-t)])."""
+    n = count_qubits(state)
+    pairs = np.asarray(qubit_pairs).reshape(-1, 2)
+    crit = 2 ** (n - pairs[:, 0] - 1) + 2 ** (n - pairs[:, 1] - 1)
+    ints = np.arange(2 ** n)
+    match = (ints[:, None] & crit[None, :]) == crit[None, :]
+    return np.prod(1 - 2 * match, axis=-1) * state
+
+
+def multi_controlled_z(qubits, state: np.ndarray) -> np.ndarray:
+    """simulation/operations.py:751-783: move the qubits last, multiply by
+    diag(1, ..., 1, -1) repeated, move them back."""
+    n = count_qubits(state)
+    qubits = list(qubits)
+    inert = [q for q in range(n) if q not in set(qubits)]
+    sub = np.ones(2 ** len(qubits))
+    sub[-1] = -1.0
+    factors = np.tile(sub, 2 ** len(inert))
+    perm = inert + qubits
+    rev = [0] * n
+    for q, i in enumerate(perm):
+        rev[i] = q
+    return permute_qubits(rev, factors * permute_qubits(perm, state))
+
+
+def multi_controlled_x(state: np.ndarray, controls, target: int) -> np.ndarray:
+    """simulation/operations.py:786-806: H on the target, C...CZ, H again."""
+    h = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    state = apply_operator(h, [target], state)
+    state = multi_controlled_z(list(controls) + [target], state)
+    return apply_operator(h, [target], state)
+
+
 # ---------------------------------------------------------------- reductions
 
 def abs_squared(state):

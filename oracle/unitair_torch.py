@@ -11,6 +11,8 @@ It replays the reference's torch calls step for step:
   apply_all_qubits operations.py:332-413 act on qubit 0, then (swap 0<->i,
                    act on qubit 0) for i = 1..n-1, roll by -1, contiguous
   apply_to_qubits  operations.py:416-503 (fuse per qubit, swap, act, permute)
+  permute / swap   operations.py:506-537, 600-654
+  multi_cz         operations.py:657-748
   reductions       states/innerprod.py:4-65
 """
 from __future__ import annotations
@@ -115,6 +117,35 @@ def apply_to_qubits(operators, qubits, state: torch.Tensor):
     sbd = st.dim() - n
     st = st.permute(list(range(sbd)) + [sbd + p for p in inv])
     return st.contiguous().reshape(st.shape[:sbd] + (N,))
+
+
+def permute_qubits(permutation, state: torch.Tensor):
+    """operations.py:600-654 (contiguous, as to_vector_layout leaves it)."""
+    N = state.shape[-1]
+    n = N.bit_length() - 1
+    st = state.reshape(state.shape[:-1] + (2,) * n)
+    sbd = st.dim() - n
+    st = st.permute(list(range(sbd)) + [sbd + p for p in permutation])
+    return st.contiguous().reshape(state.shape)
+
+
+def swap(state: torch.Tensor, qubit_pair):
+    """operations.py:506-537."""
+    n = state.shape[-1].bit_length() - 1
+    perm = list(range(n))
+    i, j = qubit_pair
+    perm[i], perm[j] = perm[j], perm[i]
+    return permute_qubits(perm, state)
+
+
+def multi_cz(qubit_pairs: torch.Tensor, state: torch.Tensor):
+    """operations.py:657-748 (batched-pairs branch)."""
+    n = state.shape[-1].bit_length() - 1
+    pairs = qubit_pairs.view(-1, 2)
+    crit = 2 ** (n - pairs[:, 0] - 1) + 2 ** (n - pairs[:, 1] - 1)
+    ints = torch.arange(2 ** n)
+    match = torch.bitwise_and(ints.view(-1, 1), crit.view(1, -1)) == crit
+    return torch.prod(1 - 2 * match, dim=-1) * state
 
 
 def abs_squared(state: torch.Tensor):

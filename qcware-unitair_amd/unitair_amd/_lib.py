@@ -37,6 +37,7 @@ DTYPE_CODES = {
 
 _i32 = ctypes.c_int
 _i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
 _p = ctypes.c_void_p
 _pi32 = ctypes.POINTER(ctypes.c_int)
 _pi64 = ctypes.POINTER(ctypes.c_int64)
@@ -50,6 +51,9 @@ SIGNATURES = {
     "ua_apply_phase": (_i32, [_p, _i32, _p, _i32, _p, _i32, _i32, _pi64, _pi64, _pi64, _p]),
     "ua_apply_all_qubits": (_i32, [_p, _p, _p, _i32, _i32, _i64, _i64, _i64, _p]),
     "ua_apply_1q_layer": (_i32, [_p, _p, _p, _i32, _i32, _i32, _pi32, _i64, _i64, _i64, _p]),
+    "ua_permute_qubits": (_i32, [_p, _p, _i32, _i32, _pi32, _i64, _i64, _p]),
+    "ua_apply_sign_gates": (_i32, [_p, _p, _i32, _i32, _i32, _pi32, _i32, _u64, _i64, _i64, _p]),
+    "ua_apply_controlled_flip": (_i32, [_p, _p, _i32, _i32, _u64, _i32, _i64, _i64, _p]),
     "ua_reduce_workspace_bytes": (_i64, [_i64, _i64]),
     "ua_norm_squared": (_i32, [_p, _i32, _p, _i32, _i64, _i64, _i64, _p, _p]),
     "ua_inner_product": (_i32, [_p, _i32, _p, _i64, _p, _i64, _i32, _i64, _i64, _p, _p]),

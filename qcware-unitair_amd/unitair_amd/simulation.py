@@ -22,7 +22,8 @@ from ._device import on_device, prepare, require_gpu, same_device
 from .states import count_qubits, count_qubits_gate_matrix
 
 __all__ = ["apply_operator", "act_first_qubits", "apply_phase", "apply_all_qubits",
-           "apply_to_qubits", "act_last_qubit"]
+           "apply_to_qubits", "act_last_qubit", "swap", "permute_qubits", "roll_qubits",
+           "multi_cz", "multi_controlled_z", "multi_controlled_x"]
 
 _SUPPORTED = tuple(_lib.DTYPE_CODES)
 
@@ -403,6 +404,173 @@ def act_last_qubit(single_qubit_operator: torch.Tensor, state: torch.Tensor) -> 
                            f"({single_qubit_operator.dim()}) for operand 0 and no ellipsis "
                            "was given")
     return apply_operator(single_qubit_operator, [n - 1], state)
+
+
+# ------------------------------------------- permutations and sign gates --
+
+def _linear_launch(kind: str, state: torch.Tensor, n: int, args) -> torch.Tensor:
+    """Out-of-place launch of one of the index-map kernels on (*batch, 2^n)."""
+    N = 2 ** n
+    s_flat, s_stride = _flat_batch(state, N)
+    batch = s_flat.shape[0]
+    out = torch.empty(tuple(state.shape[:-1]) + (N,), dtype=state.dtype, device=state.device)
+    if batch == 0:
+        return out
+    lib = _lib.load()
+    code = _lib.DTYPE_CODES[state.dtype]
+    stream = _lib.stream_handle(state.device)
+    with on_device(state.device):
+        if kind == "permute":
+            rc = lib.ua_permute_qubits(_lib.ptr(out), _lib.ptr(s_flat), code, n,
+                                       _lib.int_array(args), batch, s_stride, stream)
+        elif kind == "sign":
+            pairs, use_all, mask = args
+            flat = [b for pr in pairs for b in pr]
+            rc = lib.ua_apply_sign_gates(_lib.ptr(out), _lib.ptr(s_flat), code, n, len(pairs),
+                                         _lib.int_array(flat), use_all, mask, batch, s_stride,
+                                         stream)
+        else:
+            ctrl, tbit = args
+            rc = lib.ua_apply_controlled_flip(_lib.ptr(out), _lib.ptr(s_flat), code, n, ctrl,
+                                              tbit, batch, s_stride, stream)
+    _lib.check(rc, kind)
+    return out
+
+
+class _IndexMap(torch.autograd.Function):
+    """A permutation or +-1 diagonal of the amplitudes; the backward pass
+    applies the adjoint map (the inverse permutation; the same sign or
+    involutive flip)."""
+
+    @staticmethod
+    def forward(ctx, state, kind, n, args, adj_args):
+        ctx.kind, ctx.n, ctx.adj_args = kind, n, adj_args
+        return _linear_launch(kind, state, n, args)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _linear_launch(ctx.kind, prepare(g.contiguous()), ctx.n, ctx.adj_args), \
+            None, None, None, None
+
+
+def _index_map(state: torch.Tensor, what: str, kind: str, n: int, args, adj_args):
+    require_gpu(state, what)
+    if state.dtype not in _SUPPORTED:
+        raise TypeError(f"{what}: dtype {state.dtype} is not supported")
+    state = prepare(state)
+    if _needs_grad(state):
+        return _IndexMap.apply(state, kind, n, args, adj_args)
+    return _linear_launch(kind, state, n, args)
+
+
+def _inverse(perm: List[int]) -> List[int]:
+    inv = [0] * len(perm)
+    for i, p in enumerate(perm):
+        inv[p] = i
+    return inv
+
+
+def permute_qubits(permutation: Iterable[int], state_vector: torch.Tensor) -> torch.Tensor:
+    """Permute qubits (operations.py:600-654): output qubit q is input qubit
+    permutation[q], as torch.permute on the qubit dims of the tensor layout;
+    the same permutation for every batch entry."""
+    n = count_qubits(state_vector)
+    perm = [int(p) for p in permutation]
+    if sorted(perm) != list(range(n)):
+        raise RuntimeError(f"permute_qubits: {perm} is not a permutation of the {n} qubits")
+    return _index_map(state_vector, "permute_qubits", "permute", n, perm, _inverse(perm))
+
+
+def swap(state: torch.Tensor, qubit_pair: Tuple[int, int]) -> torch.Tensor:
+    """Swap two qubits (operations.py:506-537).  Indices follow
+    get_qubit_indices (states/shapes.py:72-116): -n <= q < n."""
+    n = count_qubits(state)
+    q1, q2 = (int(q) for q in qubit_pair)
+    for q in (q1, q2):
+        if q >= n or q < -n:
+            raise ValueError(f"Expected index in {{-num_qubits, ..., num_qubits - 1}}.\n"
+                             f"Num_qubits: {n}, index: {list(qubit_pair)}.")
+    if q1 == q2:
+        return state
+    q1, q2 = q1 % n, q2 % n
+    perm = list(range(n))
+    perm[q1], perm[q2] = perm[q2], perm[q1]
+    return _index_map(state, "swap", "permute", n, perm, perm)
+
+
+def roll_qubits(state: torch.Tensor, num_steps: int = 1) -> torch.Tensor:
+    """Cyclic permutation of the qubits (operations.py:540-597): with
+    k = num_steps mod n, output qubit i is input qubit (i - k) mod n (the
+    reference's permute(identity[-k:] + identity[:-k])), so rolling by 1
+    moves <0100| to <0010|."""
+    n = count_qubits(state)
+    k = num_steps % n
+    if k == 0:
+        return state
+    ident = list(range(n))
+    perm = ident[-k:] + ident[:-k]
+    return _index_map(state, "roll_qubits", "permute", n, perm, _inverse(perm))
+
+
+def multi_cz(qubit_pairs: torch.Tensor, state_vector: torch.Tensor,
+             num_bits_memory_cutoff=None) -> torch.Tensor:
+    """CZ on every listed pair (operations.py:657-748); pairs commute, a pair
+    listed twice cancels.  num_bits_memory_cutoff only steered the reference's
+    memory use and is accepted for compatibility."""
+    n = count_qubits(state_vector)
+    if qubit_pairs.dim() == 1:
+        qubit_pairs = qubit_pairs.view(1, 2)
+    if state_vector.device.type != qubit_pairs.device.type:
+        raise ValueError("state_vector and qubit_qubit_pairs must have same device.")
+    elif qubit_pairs.dim() != 2:
+        raise ValueError("qubit_pairs must have size (2,) or (num_pairs, 2).")
+    rows = qubit_pairs.tolist()          # one device->host copy for every check
+    if any(v >= n for r in rows for v in r):
+        raise ValueError("Control/target indices for CZ gate must be less than num_bits.")
+    if any(r[0] == r[1] for r in rows):
+        raise ValueError("Control and target qubits are not distinct.")
+    # a negative index puts its bit beyond the state, so the reference never
+    # matches that pair: it has no effect
+    pairs = [(n - 1 - int(r[0]), n - 1 - int(r[1])) for r in rows if r[0] >= 0 and r[1] >= 0]
+    args = (pairs, 0, 0)
+    return _index_map(state_vector, "multi_cz", "sign", n, args, args)
+
+
+def multi_controlled_z(qubits: Iterable[int], state_vector: torch.Tensor) -> torch.Tensor:
+    """C...CZ on `qubits` (operations.py:751-783): -1 where all of them are 1.
+    With no qubits every amplitude changes sign, as in the reference."""
+    n = count_qubits(state_vector)
+    qubits = [int(q) for q in qubits]
+    if len(set(qubits)) != len(qubits):
+        raise RuntimeError(f"multi_controlled_z: repeated qubit in {qubits}")
+    if any(q < 0 or q >= n for q in qubits):
+        raise IndexError(f"multi_controlled_z: qubits {qubits} out of range for {n} qubits")
+    mask = 0
+    for q in qubits:
+        mask |= 1 << (n - 1 - q)
+    args = ([], 1, mask)
+    return _index_map(state_vector, "multi_controlled_z", "sign", n, args, args)
+
+
+def multi_controlled_x(state_vector: torch.Tensor, controls: Iterable[int],
+                       target: int) -> torch.Tensor:
+    """C...CX (operations.py:786-806): flips `target` where every control is
+    1.  The reference computes H . C...CZ . H; here the flip is exact."""
+    n = count_qubits(state_vector)
+    controls = [int(q) for q in controls]
+    target = int(target)
+    if not 0 <= target < n:
+        raise ValueError(f"qubits={[target]} is not consistent with state vector with "
+                         f"{n} qubits.")
+    if len(set(controls + [target])) != len(controls) + 1:
+        raise RuntimeError(f"multi_controlled_x: repeated qubit in {controls + [target]}")
+    if any(q < 0 or q >= n for q in controls):
+        raise IndexError(f"multi_controlled_x: controls {controls} out of range")
+    ctrl = 0
+    for q in controls:
+        ctrl |= 1 << (n - 1 - q)
+    args = (ctrl, n - 1 - target)
+    return _index_map(state_vector, "multi_controlled_x", "flip", n, args, args)
 
 
 # ------------------------------------------------------------------- phase --

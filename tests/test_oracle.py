@@ -278,6 +278,65 @@ def test_act_last_qubit_equals_apply_to_last():
     assert np.allclose(out[0, 1], U.act_last_qubit(op, batch[0, 1]))
 
 
+def test_swap_and_permute_invariants():
+    # reference tests/test_operations.py:165-200 and tests/test_unitary.py:94-127
+    assert np.allclose(U.swap(np.array([.3, 0., 0., -0.13]), (0, 1)), [.3, 0., 0., -0.13])
+    c = np.array([2.24 + .3j, 1. + .2j, 1. + .2j, 73. - .13j])
+    assert np.allclose(U.swap(c, (0, 1)), c)
+    for n in (2, 4, 6):
+        psi = rand_c((3, 2 ** n))
+        for i, j in itertools.permutations(range(n), 2):
+            sw = U.swap(psi, (i, j))
+            assert np.array_equal(U.swap(sw, (i, j)), psi)           # involutory
+            assert np.array_equal(sw, U.swap(psi, (j, i)))           # symmetric
+            perm = list(range(n))
+            perm[i], perm[j] = j, i
+            assert np.array_equal(sw, U.permute_qubits(perm, psi))  # matches permute
+            # amplitude bookkeeping: index bits n-1-i and n-1-j exchanged
+            for x in (0, 1, 2 ** n - 2, 5 % 2 ** n):
+                bi, bj = (x >> (n - 1 - i)) & 1, (x >> (n - 1 - j)) & 1
+                y = x & ~(1 << (n - 1 - i)) & ~(1 << (n - 1 - j))
+                y |= (bj << (n - 1 - i)) | (bi << (n - 1 - j))
+                assert sw[1, y] == psi[1, x]
+
+
+def test_roll_qubits_doc_example():
+    # operations.py:547-556: rolling by one step moves <0100| to <0010|
+    psi = np.zeros(16)
+    psi[0b0100] = 1.0
+    assert U.roll_qubits(psi, 1)[0b0010] == 1.0
+    psi = rand_c(2 ** 5)
+    assert np.array_equal(U.roll_qubits(U.roll_qubits(psi, 2), -2), psi)
+    assert np.array_equal(U.roll_qubits(psi, 5), psi)
+
+
+def _controlled_x_matrix(k):
+    m = np.eye(2 ** k)
+    m[-2:, -2:] = [[0, 1], [1, 0]]
+    return m
+
+
+def test_multi_cz_and_multi_controlled_gates_closed_form():
+    n = 5
+    psi = rand_c((2, 2 ** n))
+    cz = np.diag([1, 1, 1, -1]).astype(complex)
+    pairs = np.array([[0, 3], [4, 1], [2, 3], [0, 3], [1, 2]])
+    ref = psi
+    for c, t in pairs:
+        ref = closed_form.apply_dense(cz, [c, t], ref)
+    assert np.allclose(U.multi_cz(pairs, psi), ref, atol=1e-14)
+    for qs in ([2], [0, 4], [3, 1, 2]):
+        d = np.ones(2 ** len(qs))
+        d[-1] = -1
+        assert np.allclose(U.multi_controlled_z(qs, psi),
+                           closed_form.apply_dense(np.diag(d), qs, psi), atol=1e-14)
+    assert np.allclose(U.multi_controlled_z([], psi), -psi)
+    for ctrl, t in (([0], 3), ([4, 1], 2), ([0, 1, 2], 4), ([], 1)):
+        assert np.allclose(U.multi_controlled_x(psi, ctrl, t),
+                           closed_form.apply_dense(_controlled_x_matrix(len(ctrl) + 1),
+                                                   list(ctrl) + [t], psi), atol=1e-14)
+
+
 # ------------------------------------------------------------------- phase --
 
 def basic_phase(angles, state):
@@ -343,6 +402,12 @@ def test_torch_restatement_matches_pinned_numpy_oracle():
     u = rand_c((2, 2))
     assert np.allclose(T.apply_all_qubits(torch.from_numpy(u), tp).numpy(),
                        U.apply_all_qubits(u, psi), atol=1e-12)
+    perm = [4, 1, 5, 0, 3, 2]
+    assert np.array_equal(T.permute_qubits(perm, tp).numpy(), U.permute_qubits(perm, psi))
+    assert np.array_equal(T.swap(tp, (1, 4)).numpy(), U.swap(psi, (1, 4)))
+    pairs = np.array([[0, 5], [2, 3], [0, 5], [4, 1]])
+    assert np.array_equal(T.multi_cz(torch.from_numpy(pairs), tp).numpy(),
+                          U.multi_cz(pairs, psi))
     qs = [3, 0, 3, 5]
     ops = [rand_c((2, 2)) for _ in qs]
     assert np.allclose(T.apply_to_qubits([torch.from_numpy(o) for o in ops], qs, tp).numpy(),
