@@ -320,7 +320,7 @@ def _ops_inputs(n, seed, device, dtype):
     return psi, other, u, ang, diag
 
 
-def _op_table(n, sim, states, psi, other, u, ang, diag):
+def _op_table(n, sim, states, psi, other, u, ang, diag, sample):
     """(name, callable, algorithmic bytes) for each drop-in call, complex64."""
     import torch
     A, s = 2 ** n, 8
@@ -347,6 +347,8 @@ def _op_table(n, sim, states, psi, other, u, ang, diag):
         ("swap qubits (0, n-1)", lambda: sim.swap(psi, (0, n - 1)), 2 * A * s),
         ("permute_qubits, reversed order", lambda: sim.permute_qubits(rev, psi), 2 * A * s),
         ("multi_cz, ring of n pairs", lambda: sim.multi_cz(ring, psi), 2 * A * s),
+        ("measure: draw 2^20 samples (probabilities, prefix sums, draws)",
+         lambda: sample(psi, 1 << 20), A * s + A * 8),
         ("norm_squared", lambda: states.norm_squared(psi), A * s),
         ("abs_squared", lambda: states.abs_squared(psi), A * s + A * 4),
         ("inner_product", lambda: states.inner_product(psi, other), 2 * A * s),
@@ -364,7 +366,9 @@ def ops_bench(dev, n=24, seed=5, reps=20):
     from unitair_amd import simulation as sim
     psi, other, u, ang, diag = _ops_inputs(n, seed, dev, torch.complex64)
     out = {}
-    for name, fn, nbytes in _op_table(n, sim, ua.states, psi, other, u, ang, diag):
+    from unitair_amd.measurement import sample_indices
+    for name, fn, nbytes in _op_table(n, sim, ua.states, psi, other, u, ang, diag,
+                                      sample_indices):
         fn()
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -407,7 +411,8 @@ def cpu_ops_baseline(n=24, seed=5, budget_s=10.0):
 
     out = {}
     t_start = time.perf_counter()
-    for name, fn, nbytes in _op_table(n, _Sim, _States, psi, other, u, ang, diag):
+    for name, fn, nbytes in _op_table(n, _Sim, _States, psi, other, u, ang, diag,
+                                      T.sample_indices):
         t0 = time.perf_counter()
         fn()
         us = (time.perf_counter() - t0) * 1e6

@@ -13,6 +13,7 @@ It replays the reference's torch calls step for step:
   apply_to_qubits  operations.py:416-503 (fuse per qubit, swap, act, permute)
   permute / swap   operations.py:506-537, 600-654
   multi_cz         operations.py:657-748
+  sampling         simulation/measurement.py:45-48
   reductions       states/innerprod.py:4-65
 """
 from __future__ import annotations
@@ -146,6 +147,12 @@ def multi_cz(qubit_pairs: torch.Tensor, state: torch.Tensor):
     ints = torch.arange(2 ** n)
     match = torch.bitwise_and(ints.view(-1, 1), crit.view(1, -1)) == crit
     return torch.prod(1 - 2 * match, dim=-1) * state
+
+
+def sample_indices(state: torch.Tensor, num_samples: int):
+    """measurement.py:45-48: Categorical(probs=abs_squared(state)).sample."""
+    probs = abs_squared(state)
+    return torch.distributions.Categorical(probs=probs).sample([num_samples])
 
 
 def abs_squared(state: torch.Tensor):

@@ -1,0 +1,203 @@
+// Sampling measurement outcomes: ua_sample replaces the probability and
+// draw steps of unitair.simulation.measure (unitair/simulation/measurement.py:
+// 9-71: abs_squared, then torch.distributions.Categorical(probs).sample).
+//
+// 1. scan_chunks_kernel: |psi_x|^2 in float64, inclusive prefix sums inside
+//    chunks of 4096 amplitudes (cum), and each chunk's total (chunk_sum).
+// 2. scan_totals_kernel: one workgroup turns chunk_sum into inclusive chunk
+//    prefixes (chunk_prefix).
+// 3. draw_kernel: sample s takes u = U(s) * total, U(s) a 53-bit uniform in
+//    [0, 1) from Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
+//    numbers: as easy as 1, 2, 3", SC'11) with counter (s + offset, 0, 0, 0)
+//    and key (seed lo, seed hi); the outcome is the first index whose
+//    inclusive prefix exceeds u: a binary search over chunk_prefix, then over
+//    that chunk's cum.
+// Categorical(probs) normalises by the sum of probs; so does drawing u in
+// [0, total).
+#include "ua_common.h"
+
+namespace ua {
+namespace {
+
+constexpr int kChunkBits = 12;
+constexpr int kChunk = 1 << kChunkBits;       // amplitudes per chunk
+constexpr int kScanThreads = 256;
+constexpr int kPerThread = kChunk / kScanThreads;  // 16
+
+template <typename T> __device__ __forceinline__ double prob(T v);
+template <> __device__ __forceinline__ double prob(float v) { return double(v) * double(v); }
+template <> __device__ __forceinline__ double prob(double v) { return v * v; }
+template <> __device__ __forceinline__ double prob(float2 v) {
+  return double(v.x) * double(v.x) + double(v.y) * double(v.y);
+}
+template <> __device__ __forceinline__ double prob(double2 v) { return v.x * v.x + v.y * v.y; }
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void scan_chunks_kernel(const T* state, int64_t N,
+                                                                   double* cum,
+                                                                   double* chunk_sum) {
+  __shared__ double wave_tot[kScanThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t base = int64_t(blockIdx.x) * kChunk + int64_t(tid) * kPerThread;
+  double local[kPerThread];
+  double run = 0;
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    const int64_t i = base + j;
+    run += (i < N) ? prob(state[i]) : 0.0;
+    local[j] = run;
+  }
+  // inclusive scan of the thread totals across the wavefront
+  double x = run;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wave_tot[wave] = x;
+  __syncthreads();
+  double before = x - run;  // exclusive within the wavefront
+  for (int w = 0; w < wave; ++w) before += wave_tot[w];
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    const int64_t i = base + j;
+    if (i < N) cum[i] = before + local[j];
+  }
+  if (tid == kScanThreads - 1) chunk_sum[blockIdx.x] = before + run;
+}
+
+// One workgroup: inclusive prefix of the chunk totals, 1024 at a time.
+__global__ __launch_bounds__(1024) void scan_totals_kernel(const double* chunk_sum,
+                                                          int64_t chunks, double* prefix) {
+  __shared__ double wave_tot[16];
+  __shared__ double carry_s;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) carry_s = 0;
+  __syncthreads();
+  for (int64_t start = 0; start < chunks; start += 1024) {
+    const int64_t i = start + tid;
+    const double v = i < chunks ? chunk_sum[i] : 0.0;
+    double x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) wave_tot[wave] = x;
+    __syncthreads();
+    double before = carry_s;
+    for (int w = 0; w < wave; ++w) before += wave_tot[w];
+    if (i < chunks) prefix[i] = before + x;
+    __syncthreads();
+    if (tid == 1023) carry_s = before + x;
+    __syncthreads();
+  }
+}
+
+// Philox4x32-10 as published (SC'11): 10 rounds of two 32x32->64 multiplies
+// with the Weyl key schedule.
+__device__ __forceinline__ void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = uint64_t(M0) * c[0];
+    const uint64_t p1 = uint64_t(M1) * c[2];
+    const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+    const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void draw_kernel(int64_t* out, int64_t num_samples,
+                                                     const double* cum,
+                                                     const double* prefix, int64_t chunks,
+                                                     int64_t N, uint64_t seed,
+                                                     uint64_t offset) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const double total = prefix[chunks - 1];
+  for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < num_samples;
+       s += stride) {
+    const uint64_t ctr = uint64_t(s) + offset;
+    uint32_t c[4] = {uint32_t(ctr), uint32_t(ctr >> 32), 0u, 0u};
+    philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
+    const uint64_t bits53 = (uint64_t(c[0]) << 21) ^ (uint64_t(c[1]) >> 11);
+    const double u = double(bits53 & ((uint64_t(1) << 53) - 1)) * 0x1.0p-53 * total;
+    // first chunk whose inclusive prefix exceeds u
+    int64_t lo = 0, hi = chunks - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (prefix[mid] > u) hi = mid; else lo = mid + 1;
+    }
+    const double u_local = u - (lo > 0 ? prefix[lo - 1] : 0.0);
+    const int64_t first = lo << kChunkBits;
+    int64_t a = first, b = first + kChunk - 1;
+    if (b > N - 1) b = N - 1;
+    const int64_t last = b;
+    while (a < b) {
+      const int64_t mid = (a + b) >> 1;
+      if (cum[mid] > u_local) b = mid; else a = mid + 1;
+    }
+    // rounding can leave u_local at the chunk's total: the last index
+    out[s] = a <= last ? a : last;
+  }
+}
+
+}  // namespace
+}  // namespace ua
+
+extern "C" int64_t ua_sample_workspace_bytes(int num_qubits) {
+  if (num_qubits < 0 || num_qubits > 40) return -1;
+  const int64_t N = int64_t(1) << num_qubits;
+  const int64_t chunks = (N + ua::kChunk - 1) / ua::kChunk;
+  return 8 * (N + 2 * chunks);
+}
+
+extern "C" int ua_sample(int64_t* out, const void* state, int dtype, int num_qubits,
+                         int64_t num_samples, uint64_t seed, uint64_t offset, void* workspace,
+                         ua_stream_t stream) {
+  using namespace ua;
+  UA_CHECK_ARG(num_qubits >= 0 && num_qubits <= 40, "num_qubits=%d out of range", num_qubits);
+  UA_CHECK_ARG(dtype >= UA_FLOAT32 && dtype <= UA_COMPLEX128, "bad dtype %d", dtype);
+  UA_CHECK_ARG(num_samples >= 0, "negative num_samples");
+  if (num_samples == 0) return UA_OK;
+  UA_CHECK_ARG(out && state && workspace, "NULL pointer");
+  UA_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "workspace must be 8-byte aligned");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t N = int64_t(1) << num_qubits;
+  const int64_t chunks = (N + kChunk - 1) / kChunk;
+  double* cum = static_cast<double*>(workspace);
+  double* chunk_sum = cum + N;
+  double* prefix = chunk_sum + chunks;
+  switch (dtype) {
+    case UA_FLOAT32:
+      scan_chunks_kernel<float><<<unsigned(chunks), kScanThreads, 0, s>>>(
+          static_cast<const float*>(state), N, cum, chunk_sum);
+      break;
+    case UA_FLOAT64:
+      scan_chunks_kernel<double><<<unsigned(chunks), kScanThreads, 0, s>>>(
+          static_cast<const double*>(state), N, cum, chunk_sum);
+      break;
+    case UA_COMPLEX64:
+      scan_chunks_kernel<float2><<<unsigned(chunks), kScanThreads, 0, s>>>(
+          static_cast<const float2*>(state), N, cum, chunk_sum);
+      break;
+    default:
+      scan_chunks_kernel<double2><<<unsigned(chunks), kScanThreads, 0, s>>>(
+          static_cast<const double2*>(state), N, cum, chunk_sum);
+      break;
+  }
+  if (int rc = check_launch("scan_chunks_kernel")) return rc;
+  scan_totals_kernel<<<1, 1024, 0, s>>>(chunk_sum, chunks, prefix);
+  if (int rc = check_launch("scan_totals_kernel")) return rc;
+  const int64_t blocks = (num_samples + kBlock - 1) / kBlock;
+  draw_kernel<<<unsigned(blocks < 8192 ? blocks : 8192), kBlock, 0, s>>>(
+      out, num_samples, cum, prefix, chunks, N, seed, offset);
+  return check_launch("draw_kernel");
+}

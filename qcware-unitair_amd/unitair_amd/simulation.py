@@ -23,7 +23,7 @@ from .states import count_qubits, count_qubits_gate_matrix
 
 __all__ = ["apply_operator", "act_first_qubits", "apply_phase", "apply_all_qubits",
            "apply_to_qubits", "act_last_qubit", "swap", "permute_qubits", "roll_qubits",
-           "multi_cz", "multi_controlled_z", "multi_controlled_x"]
+           "multi_cz", "multi_controlled_z", "multi_controlled_x", "measure"]
 
 _SUPPORTED = tuple(_lib.DTYPE_CODES)
 
@@ -673,3 +673,6 @@ def _phase_launch(angles: torch.Tensor, state: torch.Tensor) -> torch.Tensor:
             _lib.i64_array(ast), _lib.stream_handle(state.device))
     _lib.check(rc, "apply_phase")
     return out
+
+
+from .measurement import measure  # noqa: E402  (unitair.simulation exports measure)

@@ -134,3 +134,42 @@ def test_gate_conventions():
     assert torch.equal(gates.cnot()[2:, 2:], torch.tensor([[0, 1], [1, 0]], dtype=torch.complex64))
     u = gates.haar_unitary(8, np.random.default_rng(1))
     assert np.allclose(u.conj().T @ u, np.eye(8))
+
+
+def test_int_to_bin_and_strings():
+    # reference tests/test_utils.py:46-53
+    from unitair_amd.utils import bool_to_strings, int_to_bin
+    for num_bits in range(1, 9):
+        for x in range(2 ** num_bits):
+            digits = int_to_bin(x, num_bits=num_bits)
+            powers = torch.tensor([2 ** k for k in reversed(range(num_bits))])
+            assert x == int((digits * powers).sum())
+            assert int_to_bin(x, num_bits, str_output=True) == bool_to_strings(digits)
+    with pytest.raises(ValueError):
+        int_to_bin(8, 3)
+    with pytest.raises(ValueError):
+        int_to_bin(-1, 3)
+
+
+def test_permutation_helpers():
+    # reference tests/test_utils.py:28-44
+    from unitair_amd.utils import inverse_list_permutation, permutation_to_front
+    perm = permutation_to_front(6, [4, 1])
+    assert perm == [4, 1, 0, 2, 3, 5]
+    inv = inverse_list_permutation(perm)
+    assert [perm[i] for i in inv] == list(range(6))
+
+
+def test_measurement_histogram_host_behaviour():
+    from collections import OrderedDict
+    from unitair_amd.measurement import MeasurementHistogram
+    h = MeasurementHistogram(3, OrderedDict([("101", 7), ("000", 2)]))
+    assert h["101"] == 7 and h["111"] == 0
+    with pytest.raises(KeyError):
+        h["12"]
+    assert h.num_distinct_samples == 2 and h.observed_samples == {"101", "000"}
+    assert list(h.int_key_histogram().items()) == [(5, 7), (0, 2)]
+    assert repr(h) == "101: 7\n000: 2"
+    big = MeasurementHistogram(6, OrderedDict((format(i, "06b"), 60 - i) for i in range(55)))
+    r = repr(big)
+    assert r.count("\n") == 15 and r.endswith(" ... (40 lines omitted)")

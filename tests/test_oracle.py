@@ -337,6 +337,26 @@ def test_multi_cz_and_multi_controlled_gates_closed_form():
                                                    list(ctrl) + [t], psi), atol=1e-14)
 
 
+def test_philox_known_answers():
+    # Philox4x32-10 known-answer vectors published with the algorithm (SC'11,
+    # Random123): zero, all-ones and pi-digit counters/keys
+    from oracle.sampling import philox4x32_10
+    assert philox4x32_10((0, 0, 0, 0), (0, 0)) == (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)
+    f = 0xffffffff
+    assert philox4x32_10((f, f, f, f), (f, f)) == (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)
+    assert philox4x32_10((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344),
+                         (0xa4093822, 0x299f31d0)) == (0xd16cfe09, 0x94fdcceb, 0x5001e420,
+                                                       0x24126ea1)
+
+
+def test_draw_rule_is_the_categorical_inverse_cdf():
+    from oracle.sampling import draw, uniform53
+    p = np.array([0.0, 0.25, 0.0, 0.5, 0.25])
+    assert [draw(p, u) for u in (0.0, 0.2499, 0.25, 0.7499, 0.75, 0.999)] == [1, 1, 3, 3, 4, 4]
+    us = [uniform53(s, seed=12345) for s in range(20000)]
+    assert 0.0 <= min(us) and max(us) < 1.0 and abs(np.mean(us) - 0.5) < 0.01
+
+
 # ------------------------------------------------------------------- phase --
 
 def basic_phase(angles, state):
