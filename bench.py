@@ -240,7 +240,7 @@ def fused_dense_bench(dev, steps=3, warmup=1, seed=4, n=28):
     import numpy as np
     import torch
     from unitair_amd.circuits import CircuitRunner
-    from unitair_amd.initializations import seeded_state
+    from unitair_amd.initializations import rand_state
     from unitair_amd.workloads import fused_dense_circuit
     circ = fused_dense_circuit(seed=seed, n=n)
     amps = 2 ** n
@@ -248,7 +248,7 @@ def fused_dense_bench(dev, steps=3, warmup=1, seed=4, n=28):
     out = {"workload": f"{n} qubits, {circ.counts()}: 64 Haar 32x32 (5-qubit) + 16 Haar 16x16 "
                        f"(4-qubit) blocks on random subsets, one state, in place",
            "flops_per_step": flops}
-    base = seeded_state(n, (), seed=seed + 10, dtype=torch.complex128, device=dev)
+    base = rand_state(n, device=dev, dtype=torch.complex128, seed=seed + 10)
     for dt, name in ((torch.complex64, "c64"), (torch.complex128, "c128")):
         state = base.to(dt) if dt != torch.complex128 else base
         es = state.element_size()
@@ -459,7 +459,7 @@ def main():
     import numpy as np
     import torch
     from unitair_amd.circuits import CircuitRunner
-    from unitair_amd.initializations import seeded_state
+    from unitair_amd.initializations import rand_state
     from unitair_amd.workloads import random_circuit_24q
 
     n = args.qubits
@@ -468,7 +468,7 @@ def main():
     ngates = len(circ.gates)
     amps = 2 ** n
     dev = torch.device("cuda:0")
-    state = seeded_state(n, (), seed=args.seed + 10, dtype=torch.complex64, device=dev)
+    state = rand_state(n, device=dev, dtype=torch.complex64, seed=args.seed + 10)
     runner = CircuitRunner(circ, dtype=torch.complex64, device=dev, in_place=True)
     if args.copy_calibration_mib > 0:
         numel = args.copy_calibration_mib * (1 << 20) // 4
@@ -590,7 +590,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64",
-        "data": "synthetic (seeded Gaussian state, Haar gates, uniform phases)",
+        "data": "synthetic (seeded Gaussian state made on the device, Haar gates, uniform phases)",
         "config": {"workload": f"random circuit, {n} qubits, {ngates} gates "
                                f"({circ.counts()}), one complex64 state",
                    "qubits": n, "gates": ngates, "batch": 1},

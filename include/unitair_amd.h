@@ -172,6 +172,28 @@ int ua_sample(int64_t* out, const void* state, int dtype, int num_qubits, int64_
               uint64_t seed, uint64_t offset, void* workspace, ua_stream_t stream);
 
 /*
+ * ua_rand_state -- replaces unitair.initializations.rand_state on a device
+ * (unitair/initializations.py:71-100: torch.randn, then divide each state by
+ * its norm): batch states of 2^num_qubits Gaussian entries (real and
+ * imaginary parts independent for complex dtypes), each normalised in
+ * float64.  Philox4x32-10 keyed by `seed`, Box-Muller normals; the same seed
+ * gives the same states.  2 <= num_qubits <= 40.  workspace:
+ * ua_rand_state_workspace_bytes(num_qubits, batch, dtype) bytes.
+ */
+int64_t ua_rand_state_workspace_bytes(int num_qubits, int64_t batch, int dtype);
+int ua_rand_state(void* out, int dtype, int num_qubits, int64_t batch, uint64_t seed,
+                  void* workspace, ua_stream_t stream);
+
+/*
+ * ua_fill_state -- replaces uniform_superposition (kind 0: every entry
+ * 2^(-num_qubits/2), initializations.py:103-128) and unit_vector (kind 1: 1 at
+ * `index`, 0 elsewhere, initializations.py:46-68) on a device, for batch
+ * states of 2^num_qubits entries.
+ */
+int ua_fill_state(void* out, int dtype, int num_qubits, int64_t batch, int kind, int64_t index,
+                  ua_stream_t stream);
+
+/*
  * ua_apply_1q_layer -- replaces unitair.simulation.apply_to_qubits
  * (unitair/simulation/operations.py:416-446) and apply_to_qubits_tensor
  * (operations.py:449-503) after its per-qubit fusion

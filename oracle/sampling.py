@@ -1,4 +1,5 @@
-"""Philox4x32-10 and the draw rule of ua_sample, in Python integers.
+"""Philox4x32-10, the draw rule of ua_sample and the states of
+ua_rand_state, in Python integers and numpy.
 
 TEST INFRASTRUCTURE ONLY (the checker of the sampling kernels).
 Philox4x32-10 follows the published algorithm (Salmon, Moraes, Dror, Shaw,
@@ -41,3 +42,25 @@ def draw(probs: np.ndarray, u: float) -> int:
     """First index whose inclusive prefix sum exceeds u * sum(probs)."""
     cum = np.cumsum(probs)
     return int(np.searchsorted(cum, u * cum[-1], side="right"))
+
+
+def rand_state_np(num_qubits: int, batch: int, seed: int, complex_dtype: bool) -> np.ndarray:
+    """The states ua_rand_state makes, in float64: counter c gives four
+    Box-Muller normals (u1 = (x + 1) 2^-32, u2 = y 2^-32; r = sqrt(-2 ln u1);
+    (r cos 2 pi u2, r sin 2 pi u2)) from Philox4x32-10 with key `seed`;
+    complex entries take two normals (re, im), real entries one; each state
+    is divided by its float64 norm."""
+    N = 2 ** num_qubits
+    per_ctr = 2 if complex_dtype else 4
+    z = np.empty(batch * N * (2 if complex_dtype else 1))
+    for c in range(batch * N // per_ctr):
+        x = philox4x32_10((c & MASK, (c >> 32) & MASK, 0, 0), (seed & MASK, (seed >> 32) & MASK))
+        for h in range(2):
+            u1 = (x[2 * h] + 1.0) * 2.0 ** -32
+            u2 = x[2 * h + 1] * 2.0 ** -32
+            r = np.sqrt(-2.0 * np.log(u1))
+            z[4 * c + 2 * h] = r * np.cos(2 * np.pi * u2)
+            z[4 * c + 2 * h + 1] = r * np.sin(2 * np.pi * u2)
+    v = z[0::2] + 1j * z[1::2] if complex_dtype else z
+    v = v.reshape(batch, N)
+    return v / np.sqrt(np.sum(np.abs(v) ** 2, axis=-1, keepdims=True))

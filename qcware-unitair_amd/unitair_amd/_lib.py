@@ -51,6 +51,9 @@ SIGNATURES = {
     "ua_apply_phase": (_i32, [_p, _i32, _p, _i32, _p, _i32, _i32, _pi64, _pi64, _pi64, _p]),
     "ua_apply_all_qubits": (_i32, [_p, _p, _p, _i32, _i32, _i64, _i64, _i64, _p]),
     "ua_apply_1q_layer": (_i32, [_p, _p, _p, _i32, _i32, _i32, _pi32, _i64, _i64, _i64, _p]),
+    "ua_rand_state_workspace_bytes": (_i64, [_i32, _i64, _i32]),
+    "ua_rand_state": (_i32, [_p, _i32, _i32, _i64, _u64, _p, _p]),
+    "ua_fill_state": (_i32, [_p, _i32, _i32, _i64, _i32, _i64, _p]),
     "ua_sample_workspace_bytes": (_i64, [_i32]),
     "ua_sample": (_i32, [_p, _p, _i32, _i32, _i64, _u64, _u64, _p, _p]),
     "ua_permute_qubits": (_i32, [_p, _p, _i32, _i32, _pi32, _i64, _i64, _p]),
