@@ -117,69 +117,56 @@ __global__ __launch_bounds__(kBlock) void dense_mfma_kernel(T* out, const T* in,
       a[ti][ks] = (!im_row && !im_col) ? u.x : (!im_row ? -u.y : (!im_col ? u.y : u.x));
     }
 
-  const int64_t chunks = (p.total_groups + kGroupsPerChunk - 1) / kGroupsPerChunk;
+  const int64_t chunks = p.total_groups / kGroupsPerChunk;  // whole chunks (host-checked)
   const uint64_t gmask = (uint64_t(1) << p.log2_groups) - 1;
   // Staging order.  Each thread moves D/4 elements per chunk.
   //  gl-fastest (targets all >= bit 6): gl = tid % 64 is fixed, j = tid/64 + 4i
   //  j-fastest  (a target below bit 6): j = tid % D is fixed, gl = tid/D + (256/D) i
+  // Chunks start at multiples of 64 groups and a state holds a multiple of
+  // 64 groups, so the bits of (chunk start) and gl are disjoint and the
+  // address of element i is base(chunk) + pre[i] with
+  //   pre[i] = deposit(gl_i) + offset(j_i),
+  // computed once: no per-element index arithmetic in the loop.
   constexpr int PER = D / 4;
   constexpr int GSTEP = kBlock / D;
-  uint64_t off[PER];
-  int jj[PER];
+  int64_t pre[PER];
+  int jj[PER], glv[PER];
   const bool jfast = p.low_targets != 0;
-  const int fixed_gl = tid & (kGroupsPerChunk - 1);
-  const int fixed_j = tid & (D - 1);
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    jj[i] = jfast ? fixed_j : ((tid >> 6) + 4 * i);
-    off[i] = offset_of<K>(jj[i], p);
+    jj[i] = jfast ? (tid & (D - 1)) : ((tid >> 6) + 4 * i);
+    glv[i] = jfast ? (tid / D) + GSTEP * i : (tid & (kGroupsPerChunk - 1));
+    pre[i] = int64_t(insert_zero_bits(uint64_t(glv[i]), p.tb, K) + offset_of<K>(jj[i], p));
   }
-  auto group_gl = [&](int i) { return jfast ? (tid / D) + GSTEP * i : fixed_gl; };
-
-  // Global reads of one chunk into registers (the next chunk's reads are
-  // issued before the current chunk's matrix products, so loads and MFMA
-  // overlap inside the workgroup).
-  auto fetch = [&](int64_t chunk, T (&v)[PER]) {
+  auto base_of = [&](int64_t chunk, int64_t bstride) -> int64_t {
     const int64_t g0 = chunk * kGroupsPerChunk;
-    int64_t row_fixed = -1;
-    if (!jfast) {
-      const int64_t g = g0 + fixed_gl;
-      if (chunk < chunks && g < p.total_groups)
-        row_fixed = (g >> p.log2_groups) * p.in_bstride +
-                    int64_t(insert_zero_bits(uint64_t(g) & gmask, p.tb, K));
-    }
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int64_t g = g0 + group_gl(i);
-      int64_t row = row_fixed;
-      if (jfast) {
-        row = -1;
-        if (chunk < chunks && g < p.total_groups)
-          row = (g >> p.log2_groups) * p.in_bstride +
-                int64_t(insert_zero_bits(uint64_t(g) & gmask, p.tb, K));
-      }
-      v[i].x = 0;
-      v[i].y = 0;
-      if (row >= 0) v[i] = in[row + off[i]];
-    }
+    return (g0 >> p.log2_groups) * bstride +
+           int64_t(insert_zero_bits(uint64_t(g0) & gmask, p.tb, K));
   };
 
   T cur[PER];
   int64_t chunk = blockIdx.x;
-  fetch(chunk, cur);
+  if (chunk < chunks) {
+    const int64_t base = base_of(chunk, p.in_bstride);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) cur[i] = in[base + pre[i]];
+  }
   for (; chunk < chunks; chunk += gridDim.x) {
-    const int64_t g0 = chunk * kGroupsPerChunk;
     // ---- stage this chunk: xs[j][gl] = Re, xs[D + j][gl] = Im
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int gl = group_gl(i);
-      xs[tile_pos(jj[i], gl)] = cur[i].x;
-      xs[tile_pos(D + jj[i], gl)] = cur[i].y;
+      xs[tile_pos(jj[i], glv[i])] = cur[i].x;
+      xs[tile_pos(D + jj[i], glv[i])] = cur[i].y;
     }
     __syncthreads();
-    // ---- issue the next chunk's reads
+    // ---- issue the next chunk's reads (loads overlap the matrix products)
+    const int64_t next = chunk + gridDim.x;
     T nxt[PER];
-    fetch(chunk + gridDim.x, nxt);
+    if (next < chunks) {
+      const int64_t base = base_of(next, p.in_bstride);
+#pragma unroll
+      for (int i = 0; i < PER; ++i) nxt[i] = in[base + pre[i]];
+    }
     // ---- 16 groups per wavefront
     typename M::acc_t acc[RT];
 #pragma unroll
@@ -198,30 +185,13 @@ __global__ __launch_bounds__(kBlock) void dense_mfma_kernel(T* out, const T* in,
       for (int r = 0; r < 4; ++r) xs[tile_pos(16 * ti + M::row(lane, r), col)] = acc[ti][r];
     __syncthreads();
     // ---- store (same element order as the load)
-    int64_t orow_fixed = -1;
-    if (!jfast) {
-      const int64_t g = g0 + fixed_gl;
-      if (g < p.total_groups)
-        orow_fixed = (g >> p.log2_groups) * p.out_bstride +
-                     int64_t(insert_zero_bits(uint64_t(g) & gmask, p.tb, K));
-    }
+    const int64_t obase = base_of(chunk, p.out_bstride);
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int gl = group_gl(i);
-      const int64_t g = g0 + gl;
-      int64_t row = orow_fixed;
-      if (jfast) {
-        row = -1;
-        if (g < p.total_groups)
-          row = (g >> p.log2_groups) * p.out_bstride +
-                int64_t(insert_zero_bits(uint64_t(g) & gmask, p.tb, K));
-      }
-      if (row >= 0) {
-        T v;
-        v.x = xs[tile_pos(jj[i], gl)];
-        v.y = xs[tile_pos(D + jj[i], gl)];
-        out[row + off[i]] = v;
-      }
+      T v;
+      v.x = xs[tile_pos(jj[i], glv[i])];
+      v.y = xs[tile_pos(D + jj[i], glv[i])];
+      out[obase + pre[i]] = v;
     }
 #pragma unroll
     for (int i = 0; i < PER; ++i) cur[i] = nxt[i];
@@ -232,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void dense_mfma_kernel(T* out, const T* in,
 template <typename T>
 int launch(int k, void* out, const void* in, const void* op, const MfmaParams& p,
            hipStream_t s) {
-  const int64_t chunks = (p.total_groups + kGroupsPerChunk - 1) / kGroupsPerChunk;
+  const int64_t chunks = p.total_groups / kGroupsPerChunk;
   int64_t blocks = chunks < 2048 ? chunks : 2048;
   if (blocks < 1) blocks = 1;
   if (k == 4)
@@ -261,6 +231,8 @@ int apply_dense_mfma(void* out, const void* in, const void* op, int dtype, int n
     p.gpos[m] = gp[m];
   }
   p.low_targets = tb[0] < 6 ? 1 : 0;
+  // whole chunks of 64 groups, none across two states
+  UA_CHECK_ARG(n - k >= 6, "matrix-core path needs n - k >= 6");
   if (dtype == UA_COMPLEX64) return launch<float2>(k, out, in, op, p, s);
   return launch<double2>(k, out, in, op, p, s);
 }
