@@ -217,6 +217,10 @@ def serving_bench(dev, num=24, seed=11):
     cap_reqs = poisson_workload(num, rate=0.0, seed=seed)
     run_one_at_a_time(cap_reqs, device=dev, realtime=False)
     capacity = num / max(r.done for r in cap_reqs)
+    # the same requests all at once through the continuous-batching server
+    sat_reqs = poisson_workload(num, rate=0.0, seed=seed)
+    Server(device=dev).run(sat_reqs, realtime=False)
+    capacity_cb = num / max(r.done for r in sat_reqs)
     rate = 0.7 * capacity
     served = Server(device=dev).run(poisson_workload(num, rate=rate, seed=seed), realtime=True)
     plain = run_one_at_a_time(poisson_workload(num, rate=rate, seed=seed), device=dev,
@@ -224,7 +228,9 @@ def serving_bench(dev, num=24, seed=11):
     same = max(max(abs(a.z0 - b.z0), abs(a.norm2 - b.norm2)) for a, b in zip(served, plain))
     return {"workload": f"{num} requests, n log-uniform [10,26], depth U[10,200] layers of "
                         f"Haar 1q on every qubit + CZ on n//2 adjacent pairs, complex64",
-            "one_at_a_time_capacity_circuits_per_s": capacity, "arrival_rate_per_s": rate,
+            "one_at_a_time_capacity_circuits_per_s": capacity,
+            "continuous_batching_capacity_circuits_per_s": capacity_cb,
+            "capacity_ratio": capacity_cb / capacity, "arrival_rate_per_s": rate,
             "continuous_batching": summarize(served), "one_at_a_time": summarize(plain),
             "max_result_difference": same}
 
