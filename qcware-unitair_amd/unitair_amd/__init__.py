@@ -10,6 +10,7 @@ from . import states
 from . import simulation
 from . import gates
 from . import initializations
+from . import utils
 
 from .states import StateLayout, StateShapeError
 from .states import count_qubits, hilbert_space_dim

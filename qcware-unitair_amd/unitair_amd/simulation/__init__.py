@@ -17,9 +17,9 @@ from typing import Iterable, List, Sequence, Tuple
 
 import torch
 
-from . import _lib
-from ._device import on_device, prepare, require_gpu, same_device
-from .states import count_qubits, count_qubits_gate_matrix
+from .. import _lib
+from .._device import on_device, prepare, require_gpu, same_device
+from ..states import count_qubits, count_qubits_gate_matrix
 
 __all__ = ["apply_operator", "act_first_qubits", "apply_phase", "apply_all_qubits",
            "apply_to_qubits", "act_last_qubit", "swap", "permute_qubits", "roll_qubits",
@@ -335,7 +335,7 @@ def apply_to_qubits(operators: Iterable[torch.Tensor], qubits: Iterable[int],
     entry by entry; other broadcasts apply the fused gates one by one with the
     reference's apply_operator rules.
     """
-    from .gates.matrix_algebra import fuse_single_qubit_operators
+    from ..gates.matrix_algebra import fuse_single_qubit_operators
     n = count_qubits(state)
     fused = fuse_single_qubit_operators(qubits, operators)
     for q in fused:
@@ -675,4 +675,7 @@ def _phase_launch(angles: torch.Tensor, state: torch.Tensor) -> torch.Tensor:
     return out
 
 
-from .measurement import measure  # noqa: E402  (unitair.simulation exports measure)
+from ..measurement import measure  # noqa: E402  (unitair.simulation exports measure)
+
+# submodules with the reference's layout (unitair/simulation/{operations,measurement,utils}.py)
+from . import operations, measurement, utils  # noqa: E402,F401

@@ -13,8 +13,8 @@ from typing import List, Tuple, Union
 
 import torch
 
-from . import _lib
-from ._device import require_gpu, same_device, prepare
+from .. import _lib
+from .._device import require_gpu, same_device, prepare
 
 __all__ = [
     "StateLayout", "StateShapeError", "count_qubits", "count_qubits_gate_matrix",
@@ -252,3 +252,7 @@ def diag_expectation_value(diag_values: torch.Tensor, state: torch.Tensor) -> to
     if out_dtype not in _lib.DTYPE_CODES:
         raise TypeError(f"diag_expectation_value: unsupported result dtype {out_dtype}")
     return _reduce("diag", out_dtype, d, s)
+
+
+# submodules with the reference's layout (unitair/states/{shapes,conversions,innerprod}.py)
+from . import shapes, conversions, innerprod  # noqa: E402,F401

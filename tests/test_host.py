@@ -173,3 +173,45 @@ def test_measurement_histogram_host_behaviour():
     big = MeasurementHistogram(6, OrderedDict((format(i, "06b"), 60 - i) for i in range(55)))
     r = repr(big)
     assert r.count("\n") == 15 and r.endswith(" ... (40 lines omitted)")
+
+
+def test_reference_module_paths():
+    # the reference's package layout: unitair.states.{shapes,conversions,innerprod},
+    # unitair.simulation.{operations,measurement,utils}, unitair.gates.{gates,matrix_algebra}
+    import importlib
+    for mod, names in {
+        "unitair_amd.states.shapes": ["StateLayout", "StateShapeError", "count_qubits",
+                                      "count_qubits_gate_matrix", "get_qubit_indices",
+                                      "subset_roll_to_back", "subset_roll_to_front"],
+        "unitair_amd.states.conversions": ["to_tensor_layout", "to_vector_layout"],
+        "unitair_amd.states.innerprod": ["abs_squared", "norm_squared", "inner_product",
+                                         "diag_expectation_value"],
+        "unitair_amd.simulation.operations": ["apply_operator", "apply_phase", "apply_all_qubits",
+                                              "apply_to_qubits", "act_first_qubits",
+                                              "act_last_qubit", "swap", "permute_qubits",
+                                              "roll_qubits", "multi_cz", "multi_controlled_z",
+                                              "multi_controlled_x"],
+        "unitair_amd.simulation.measurement": ["measure", "MeasurementHistogram"],
+        "unitair_amd.simulation.utils": ["count_gate_batch_dims", "gate_batch_size"],
+        "unitair_amd.gates.gates": ["exp_x", "exp_y", "exp_z", "hadamard", "pauli_x", "pauli_y",
+                                    "pauli_z", "cnot", "cz"],
+        "unitair_amd.gates.matrix_algebra": ["fuse_single_qubit_operators"],
+        "unitair_amd.utils": ["int_to_bin", "bool_to_strings", "permutation_to_front",
+                              "inverse_list_permutation"],
+        "unitair_amd.simulation": ["apply_operator", "apply_phase", "apply_to_qubits",
+                                   "apply_all_qubits", "act_last_qubit", "swap", "multi_cz",
+                                   "multi_controlled_x", "multi_controlled_z",
+                                   "act_first_qubits", "measure"],
+        "unitair_amd": ["states", "simulation", "StateLayout", "unit_vector",
+                        "uniform_superposition", "rand_state", "count_qubits",
+                        "hilbert_space_dim", "diag_expectation_value", "inner_product",
+                        "norm_squared", "abs_squared", "VECTOR_LAYOUT", "TENSOR_LAYOUT"],
+    }.items():
+        m = importlib.import_module(mod)
+        for name in names:
+            assert hasattr(m, name), (mod, name)
+    from unitair_amd.simulation.utils import count_gate_batch_dims, gate_batch_size
+    assert count_gate_batch_dims(torch.zeros(3, 5, 4, 4)) == 2
+    assert gate_batch_size(torch.zeros(3, 5, 4, 4)) == torch.Size([3, 5])
+    with pytest.raises(RuntimeError):
+        count_gate_batch_dims(torch.zeros(4))
