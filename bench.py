@@ -61,7 +61,7 @@ def time_steps(runner, state, steps, warmup, with_events):
     from unitair_amd._hip import EventPool
     for _ in range(warmup):
         runner.run(state)
-    pools = [EventPool(len(runner.kinds)) for _ in range(steps)] if with_events else None
+    pools = [EventPool(len(runner.kinds), shared=True) for _ in range(steps)] if with_events else None
     if pools is None:
         pools = [None] * steps
     torch.cuda.synchronize()

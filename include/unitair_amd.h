@@ -286,7 +286,8 @@ typedef struct {
  * Otherwise every gate reads one buffer and writes the other (the work of one
  * out-of-place apply_operator call per gate); *result_in_work (if not NULL)
  * tells which buffer holds the result.  events: NULL, or 2*num_gates
- * hipEvent_t handles recorded on `stream` before and after each gate.
+ * hipEvent_t handles recorded on `stream` before and after each gate; where
+ * gate i's start handle equals gate i-1's end handle it is recorded once.
  */
 int ua_run_circuit(void* state, void* work, int dtype, int num_qubits, int64_t batch,
                    const ua_gate* gates, int num_gates, void* const* events,
@@ -314,9 +315,14 @@ enum {
   UA_FOP_DIAG1 = 4,  /* a: state index bit; data: (d0, d1), state dtype      */
   UA_FOP_PHASE = 5,  /* data: 2**n real angles, exp(-1j*angle) per element
                         (float32 for complex64 states, float64 for complex128) */
-  UA_FOP_CZ_LAYER = 6 /* data: uint64 mask(s); bit j set = CZ between index bits
+  UA_FOP_CZ_LAYER = 6, /* data: uint64 mask(s); bit j set = CZ between index bits
                         j and j+1 (qubits n-2-j and n-1-j); data_bstride in
                         masks (1 = one mask per state) */
+  UA_FOP_LAYER1Q = 7  /* a: four 8-bit counts c0..c3 (c_k in bits 8k..8k+7); the
+                        next c0+c1+c2+c3 ops are UA_FOP_DENSE1 on cube bit 0
+                        (c0 of them, in order), then cube bit 1, 2, 3: 1-qubit
+                        gates on distinct bits commute, so a run of them is
+                        applied grouped by bit with no per-op dispatch */
 };
 
 #define UA_FUSED_MATRIX_CACHE 1024 /* elements of gate data a pass keeps on chip */

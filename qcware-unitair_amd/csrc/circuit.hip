@@ -35,7 +35,8 @@ extern "C" int ua_run_circuit(void* state, void* work, int dtype, int num_qubits
   for (int i = 0; i < num_gates; ++i) {
     const ua_gate& g = gates[i];
     void* dst = (work == nullptr) ? cur : (cur == state ? work : state);
-    if (events) {
+    // a start event that is the previous gate's end event is already recorded
+    if (events && !(i > 0 && events[2 * i] == events[2 * i - 1])) {
       const hipError_t e = hipEventRecord(static_cast<hipEvent_t>(events[2 * i]), s);
       if (e != hipSuccess) {
         set_error("hipEventRecord: %s", hipGetErrorString(e));

@@ -317,6 +317,25 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
             }
             break;
           }
+          case UA_FOP_LAYER1Q: {  // a: four 8-bit counts; DENSE1 sub-ops follow
+            const uint32_t counts = uint32_t(op.a);
+            int sub = o + 1;
+#define UA_LAYER_BIT(B)                                                             \
+            for (int t = 0, nb = int((counts >> (8 * B)) & 255u); t < nb; ++t, ++sub) { \
+              const ua_fused_op so = ops[sub];                                      \
+              const T* u = so.cache_slot >= 0                                       \
+                               ? mcache + so.cache_slot                             \
+                               : static_cast<const T*>(so.data) + b * so.data_bstride; \
+              cube_1q<B>(c, u);                                                     \
+            }
+            UA_LAYER_BIT(0)
+            UA_LAYER_BIT(1)
+            UA_LAYER_BIT(2)
+            UA_LAYER_BIT(3)
+#undef UA_LAYER_BIT
+            o = sub - 1;
+            break;
+          }
           case UA_FOP_PHASE: {  // data: 2^n real angles per state
             const R* th = static_cast<const R*>(op.data) + b * op.data_bstride;
 #pragma unroll
@@ -378,9 +397,23 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
   UA_CHECK_ARG(pass->stage_ops[0] == 0, "stage_ops[0] must be 0");
   if (host_ops) {
     const int nops = pass->stage_ops[pass->num_stages];
+    // a layer op's sub-ops must be DENSE1 on the counted cube bits, inside its stage
+    for (int st = 0; st < pass->num_stages; ++st)
+      for (int o = pass->stage_ops[st]; o < pass->stage_ops[st + 1]; ++o) {
+        if (host_ops[o].kind != UA_FOP_LAYER1Q) continue;
+        const uint32_t counts = uint32_t(host_ops[o].a);
+        int sub = o + 1;
+        for (int bit = 0; bit < 4; ++bit)
+          for (uint32_t t = 0; t < ((counts >> (8 * bit)) & 255u); ++t, ++sub)
+            UA_CHECK_ARG(sub < pass->stage_ops[st + 1] && host_ops[sub].kind == UA_FOP_DENSE1 &&
+                             host_ops[sub].a == bit,
+                         "op %d: layer sub-op %d is not DENSE1 on cube bit %d of its stage", o,
+                         sub, bit);
+        o = sub - 1;
+      }
     for (int o = 0; o < nops; ++o) {
       const ua_fused_op& op = host_ops[o];
-      UA_CHECK_ARG(op.kind >= UA_FOP_DENSE1 && op.kind <= UA_FOP_CZ_LAYER, "op %d: bad kind", o);
+      UA_CHECK_ARG(op.kind >= UA_FOP_DENSE1 && op.kind <= UA_FOP_LAYER1Q, "op %d: bad kind", o);
       if (op.kind == UA_FOP_DENSE1 || op.kind == UA_FOP_CNOT)
         UA_CHECK_ARG(op.a >= 0 && op.a < (op.kind == UA_FOP_CNOT ? n : 4) &&
                          (op.kind == UA_FOP_DENSE1 || (op.b >= 0 && op.b < 4)),
@@ -394,11 +427,12 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
       if (op.cache_slot >= 0) {
         const int len = op.kind == UA_FOP_DENSE1 ? 4 : op.kind == UA_FOP_DENSE2 ? 16 : 2;
         UA_CHECK_ARG(op.kind != UA_FOP_CZ && op.kind != UA_FOP_CNOT && op.kind != UA_FOP_PHASE &&
-                         op.kind != UA_FOP_CZ_LAYER &&
+                         op.kind != UA_FOP_CZ_LAYER && op.kind != UA_FOP_LAYER1Q &&
                          op.data_bstride == 0 && op.cache_slot + len <= kMatCache,
                      "op %d: bad cache slot %d", o, op.cache_slot);
       }
-      UA_CHECK_ARG(op.data || op.kind == UA_FOP_CZ || op.kind == UA_FOP_CNOT,
+      UA_CHECK_ARG(op.data || op.kind == UA_FOP_CZ || op.kind == UA_FOP_CNOT ||
+                       op.kind == UA_FOP_LAYER1Q,
                    "op %d: NULL data", o);
     }
   }

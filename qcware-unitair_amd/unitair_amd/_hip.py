@@ -29,17 +29,25 @@ def runtime() -> ctypes.CDLL:
 
 
 class EventPool:
-    """2*n hipEvent_t handles for per-gate (start, end) timing."""
+    """2*n hipEvent_t handles for per-gate (start, end) timing.  shared=True:
+    gate i's start is gate i-1's end (n + 1 events, recorded once each), so a
+    gate's time also holds the gap before it."""
 
-    def __init__(self, n_gates: int):
+    def __init__(self, n_gates: int, shared: bool = False):
         rt = runtime()
         self.n = n_gates
         self.handles = (ctypes.c_void_p * (2 * n_gates))()
-        for i in range(2 * n_gates):
+        count = n_gates + 1 if shared else 2 * n_gates
+        made = []
+        for _ in range(count):
             h = ctypes.c_void_p()
             if rt.hipEventCreate(ctypes.byref(h)) != 0:
                 raise RuntimeError("hipEventCreate failed")
-            self.handles[i] = h
+            made.append(h.value)
+        for i in range(n_gates):
+            self.handles[2 * i] = made[i] if shared else made[2 * i]
+            self.handles[2 * i + 1] = made[i + 1] if shared else made[2 * i + 1]
+        self._made = made
         self._alive = True
 
     def elapsed_ms(self):
@@ -57,6 +65,6 @@ class EventPool:
 
     def __del__(self):
         if getattr(self, "_alive", False) and _rt is not None:
-            for h in self.handles:
-                _rt.hipEventDestroy(h)
+            for h in self._made:
+                _rt.hipEventDestroy(ctypes.c_void_p(h))
             self._alive = False

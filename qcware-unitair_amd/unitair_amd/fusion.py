@@ -29,7 +29,8 @@ TILE_BITS = 12
 CUBE_BITS = 4
 MAX_STAGES = _lib.UA_FUSED_MAX_STAGES
 
-FOP_DENSE1, FOP_DENSE2, FOP_CNOT, FOP_CZ, FOP_DIAG1, FOP_PHASE, FOP_CZ_LAYER = range(7)
+(FOP_DENSE1, FOP_DENSE2, FOP_CNOT, FOP_CZ, FOP_DIAG1, FOP_PHASE, FOP_CZ_LAYER,
+ FOP_LAYER1Q) = range(8)
 
 
 FusedOp = _lib.FusedOp
@@ -74,6 +75,65 @@ def _merge_1q(items: List[_Item]) -> List[_Item]:
             for bt in touched:
                 last_on_bit.pop(bt, None)
     return out
+
+
+def _as_dense1(it: _Item) -> _Item:
+    """A 1-qubit diagonal gate as a 2x2 matrix (host or per-state device data)."""
+    if it.kind == "dense1":
+        return it
+    d = it.data
+    m = torch.diag_embed(d).contiguous() if isinstance(d, torch.Tensor) else np.diag(np.asarray(d))
+    return _Item("dense1", it.bits, m, it.per_state)
+
+
+def _product(later: _Item, earlier: _Item) -> _Item:
+    """later @ earlier for two 1-qubit gates on the same bit; per-state data
+    (device tensors, one matrix per state) is multiplied state by state."""
+    a, b = later.data, earlier.data
+    if isinstance(a, torch.Tensor) or isinstance(b, torch.Tensor):
+        dev = a.device if isinstance(a, torch.Tensor) else b.device
+        dt = a.dtype if isinstance(a, torch.Tensor) else b.dtype
+        a = torch.as_tensor(a).to(dev, dt)
+        b = torch.as_tensor(b).to(dev, dt)
+        return _Item("dense1", later.bits, torch.matmul(a, b).contiguous(),
+                     later.per_state or earlier.per_state)
+    return _Item("dense1", later.bits, np.asarray(a) @ np.asarray(b))
+
+
+def _segment_stage(ops: List[_Item], cube_state_bits) -> list:
+    """Rewrite one stage's gate list for the kernel: every maximal run of
+    1-qubit gates on the stage's cube bits (diagonal ones included, as 2x2
+    matrices) becomes ("layer", [[gates on cube bit 0], ..., [on bit 3]]),
+    with consecutive gates on one bit multiplied together; 1-qubit gates on
+    distinct bits commute, so the run can be applied bit by bit.  Other
+    gates stay ("op", item) in place."""
+    cube_pos = {sb: k for k, sb in enumerate(cube_state_bits)}
+    out, run = [], None
+    for it in ops:
+        if it.kind in ("dense1", "diag1") and it.bits[0] in cube_pos:
+            if run is None:
+                run = [[] for _ in range(len(cube_state_bits))]
+            lane = run[cube_pos[it.bits[0]]]
+            g = _as_dense1(it)
+            if lane:
+                lane[-1] = _product(g, lane[-1])
+            else:
+                lane.append(g)
+            continue
+        if run is not None:
+            out.append(("layer", run))
+            run = None
+        out.append(("op", it))
+    if run is not None:
+        out.append(("layer", run))
+    # a run that is a single gate is cheaper as a plain op
+    flat = []
+    for kind, x in out:
+        if kind == "layer" and sum(len(l) for l in x) == 1:
+            flat.append(("op", next(g for l in x for g in l)))
+        else:
+            flat.append((kind, x))
+    return flat
 
 
 def _items_from_circuit(circ: Circuit) -> List[_Item]:
@@ -228,7 +288,22 @@ class FusedProgram:
                         fp.stage_bits[si][k] = cube[k]
                     fp.stage_ops[si] = len(op_structs) - first_op
                     pos = {t: i for i, t in enumerate(cube)}
-                    for it in ops:
+                    entries = _segment_stage(ops, [p.local[t] for t in cube])
+                    flat_ops = []
+                    for kind, x in entries:
+                        if kind == "op":
+                            flat_ops.append(x)
+                            continue
+                        head = FusedOp()
+                        head.kind = FOP_LAYER1Q
+                        head.a = sum(len(lane) << (8 * k) for k, lane in enumerate(x))
+                        flat_ops.append(head)
+                        for lane in x:
+                            flat_ops.extend(lane)
+                    for it in flat_ops:
+                        if isinstance(it, FusedOp):       # a layer head
+                            op_structs.append(it)
+                            continue
                         o = FusedOp()
                         if it.kind == "dense1":
                             o.kind, o.a = FOP_DENSE1, pos[tile_of[it.bits[0]]]

@@ -95,3 +95,52 @@ def test_24q_circuit_needs_few_passes():
     items = fusion._merge_1q(fusion._items_from_circuit(circ))
     plan = fusion.plan_passes(items, 24)
     assert len(plan) < 40
+
+
+def test_stage_segments_keep_the_stage_result():
+    # _segment_stage groups runs of 1-qubit gates on cube bits by bit (and
+    # multiplies same-bit neighbours); applying its output in kernel order
+    # must give the original stage's result
+    from unitair_amd.fusion import _Item, _segment_stage
+    from oracle import closed_form
+    rng = np.random.default_rng(12)
+    n = 6
+
+    def haar2():
+        q, r = np.linalg.qr(rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2)))
+        return q * (np.diag(r) / abs(np.diag(r)))
+
+    cube = [0, 2, 3, 5]                              # state bits of the cube
+    items = [_Item("dense1", (2,), haar2()), _Item("diag1", (0,), np.exp(1j * rng.random(2))),
+             _Item("dense1", (5,), haar2()), _Item("dense1", (2,), haar2()),
+             _Item("cz", (2, 4)),
+             _Item("dense1", (3,), haar2()), _Item("diag1", (1,), np.exp(1j * rng.random(2))),
+             _Item("dense1", (0,), haar2()), _Item("dense2", (3, 0), np.linalg.qr(
+                 rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0]),
+             _Item("dense1", (5,), haar2())]
+
+    def apply(its, psi):
+        for it in its:
+            qs = [n - 1 - b for b in it.bits]
+            if it.kind == "dense1":
+                psi = closed_form.apply_dense(it.data, qs, psi)
+            elif it.kind == "diag1":
+                psi = closed_form.apply_dense(np.diag(it.data), qs, psi)
+            elif it.kind == "dense2":
+                psi = closed_form.apply_dense(it.data, qs, psi)
+            elif it.kind == "cz":
+                psi = closed_form.apply_dense(np.diag([1, 1, 1, -1]), qs, psi)
+        return psi
+
+    seg = _segment_stage(items, cube)
+    kinds = [k for k, _ in seg]
+    # runs: {2,0,5,2} -> layer; cz; {3, diag on bit 1 (not a cube bit) ends the run};
+    # diag1 bit 1 stays an op; {0} alone -> plain op; dense2; {5} alone -> plain op
+    assert kinds == ["layer", "op", "op", "op", "op", "op", "op"]
+    layer = seg[0][1]
+    assert [len(l) for l in layer] == [1, 1, 0, 1]   # bit 2 (cube pos 1) merged to one gate
+    flat = []
+    for k, x in seg:
+        flat.extend([g for lane in x for g in lane] if k == "layer" else [x])
+    psi = rng.standard_normal(2 ** n) + 1j * rng.standard_normal(2 ** n)
+    assert np.allclose(apply(flat, psi), apply(items, psi), atol=1e-13)
