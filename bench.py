@@ -42,6 +42,8 @@ def parse():
                    help="skip the 28-qubit dense 4/5-qubit block workload")
     p.add_argument("--no-graph", action="store_true", help="skip the graph-replay leg")
     p.add_argument("--no-ops", action="store_true", help="skip the per-call drop-in leg")
+    p.add_argument("--no-stream-ref", action="store_true",
+                   help="skip timing torch's own streaming kernels on the same state size")
     p.add_argument("--no-fused", action="store_true", help="skip the fused-pass leg")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="run one float32 multiply-by-one of this many MiB (read + write) "
@@ -428,6 +430,30 @@ def cpu_ops_baseline(n=24, seed=5, budget_s=10.0):
     return out
 
 
+def stream_reference(dev, n, reps=50):
+    """What plain streaming reaches on this GPU at the state's size: torch's
+    in-place elementwise multiply and its device-to-device copy of one
+    n-qubit complex64 state (read + write of 2^n x 8 B each)."""
+    import torch
+    x = torch.ones(2 ** n, dtype=torch.complex64, device=dev)
+    y = torch.empty_like(x)
+    nbytes = 2 * x.numel() * 8
+    out = {}
+    for name, fn in (("torch_inplace_mul", lambda: x.mul_(1.0)), ("torch_copy", lambda: y.copy_(x))):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / reps * 1e3
+        out[name] = {"us": us, "GBps": nbytes / us / 1e3}
+    return out
+
+
 def measured_traffic():
     """HBM-side bytes per launch of the 1-qubit dense kernels at 24 qubits,
     from the FETCH_SIZE / WRITE_SIZE counter passes (scaled by a copy kernel
@@ -558,6 +584,11 @@ def main():
             f"{sv['continuous_batching']['circuits_per_s']:.1f} circ/s; one-at-a-time p50 "
             f"{sv['one_at_a_time']['p50_ms']:.1f} p99 {sv['one_at_a_time']['p99_ms']:.1f} ms, "
             f"{sv['one_at_a_time']['circuits_per_s']:.1f} circ/s")
+
+    if not args.no_stream_ref:
+        extra["stream_reference"] = stream_reference(dev, n)
+        log("stream reference: " + ", ".join(f"{k} {v['GBps']:.0f} GB/s"
+                                              for k, v in extra["stream_reference"].items()))
 
     if not args.no_fused_dense:
         log("fused dense 4/5-qubit blocks, 28 qubits")
