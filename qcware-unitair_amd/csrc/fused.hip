@@ -306,11 +306,29 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
             break;
           }
           case UA_FOP_CZ_LAYER: {  // data: one uint64 mask per state
+            // Sign of entry j: parity of popcount(g & (g >> 1) & mask) with
+            // g = gbase + cube part x (disjoint bits).  Over GF(2) it splits
+            // into P(gbase), a term linear in the cube bits (a cube bit at p
+            // pairs with gbase bits p-1 and p+1) and a term for cube bits
+            // that are neighbours; built as a 16-bit mask, no per-entry
+            // 64-bit index.
             const uint64_t mask = static_cast<const uint64_t*>(op.data)[b * op.data_bstride];
+            const int cbk[4] = {cb0, cb1, cb2, cb3};
+            const uint32_t selk[4] = {0xAAAAu, 0xCCCCu, 0xF0F0u, 0xFF00u};
+            uint32_t neg = (__popcll(gbase & (gbase >> 1) & mask) & 1) ? 0xFFFFu : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const int pk = cbk[k];
+              uint64_t l = (gbase >> (pk + 1)) & (mask >> pk);
+              if (pk > 0) l ^= (gbase >> (pk - 1)) & (mask >> (pk - 1));
+              if (l & 1) neg ^= selk[k];
+#pragma unroll
+              for (int k2 = 0; k2 < 4; ++k2)
+                if (cbk[k2] == pk + 1 && ((mask >> pk) & 1)) neg ^= selk[k] & selk[k2];
+            }
 #pragma unroll
             for (int j = 0; j < kCube; ++j) {
-              const uint64_t g = gidx(j);
-              if (__popcll(g & (g >> 1) & mask) & 1) {
+              if ((neg >> j) & 1) {
                 c[j].x = -c[j].x;
                 c[j].y = -c[j].y;
               }
