@@ -115,11 +115,23 @@ __device__ __forceinline__ void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uin
   }
 }
 
+// LDS_PREFIX: the chunk prefixes (at most kLdsChunks) are staged in LDS once
+// per workgroup, so the first half of every search stays on chip.
+constexpr int kLdsChunks = 4096;  // 32 KiB of doubles (states up to 24 qubits)
+
+template <bool LDS_PREFIX>
 __global__ __launch_bounds__(kBlock) void draw_kernel(int64_t* out, int64_t num_samples,
                                                      const double* cum,
-                                                     const double* prefix, int64_t chunks,
+                                                     const double* gprefix, int64_t chunks,
                                                      int64_t N, uint64_t seed,
                                                      uint64_t offset) {
+  extern __shared__ double sprefix[];
+  const double* prefix = gprefix;
+  if constexpr (LDS_PREFIX) {
+    for (int64_t i = threadIdx.x; i < chunks; i += blockDim.x) sprefix[i] = gprefix[i];
+    __syncthreads();
+    prefix = sprefix;
+  }
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   const double total = prefix[chunks - 1];
   for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < num_samples;
@@ -197,7 +209,14 @@ extern "C" int ua_sample(int64_t* out, const void* state, int dtype, int num_qub
   scan_totals_kernel<<<1, 1024, 0, s>>>(chunk_sum, chunks, prefix);
   if (int rc = check_launch("scan_totals_kernel")) return rc;
   const int64_t blocks = (num_samples + kBlock - 1) / kBlock;
-  draw_kernel<<<unsigned(blocks < 8192 ? blocks : 8192), kBlock, 0, s>>>(
-      out, num_samples, cum, prefix, chunks, N, seed, offset);
+  if (chunks <= kLdsChunks) {
+    // enough workgroups to fill the GPU, each drawing many samples
+    const unsigned grid = unsigned(blocks < 1024 ? blocks : 1024);
+    draw_kernel<true><<<grid, kBlock, size_t(chunks) * sizeof(double), s>>>(
+        out, num_samples, cum, prefix, chunks, N, seed, offset);
+  } else {
+    draw_kernel<false><<<unsigned(blocks < 8192 ? blocks : 8192), kBlock, 0, s>>>(
+        out, num_samples, cum, prefix, chunks, N, seed, offset);
+  }
   return check_launch("draw_kernel");
 }

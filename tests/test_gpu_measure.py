@@ -110,3 +110,13 @@ def test_measure_rejects_batches_and_cpu(device):
         sim.measure(torch.ones(2, 8, dtype=torch.complex64, device=device), 10)
     with pytest.raises(RuntimeError):
         sim.measure(torch.ones(8, dtype=torch.complex64), 10)
+
+
+def test_uniform_weights_25_qubits_global_prefix_search(device):
+    # 8192 chunk prefixes: the search over them stays in global memory
+    n = 25
+    state = torch.ones(2 ** n, dtype=torch.complex64, device=device)
+    k = 3000
+    got = sample_indices(state, k, generator=torch.Generator().manual_seed(21)).cpu().numpy()
+    seed = _seed(21)
+    assert np.array_equal(got, [int(uniform53(s, seed) * 2 ** n) for s in range(k)])
