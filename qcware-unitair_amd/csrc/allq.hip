@@ -242,8 +242,8 @@ int layer_impl(void* out, const void* state, const void* op, int dtype, int n,
                int num_targets, const int* qubits, int64_t batch, int64_t state_bstride,
                int64_t op_bstride, int64_t target_stride, hipStream_t s) {
   UA_CHECK_ARG(n >= 1 && n <= 62, "1-qubit layers need 1..62 qubits (got %d)", n);
-  UA_CHECK_ARG(out != state, "1-qubit layers are out of place");
   UA_CHECK_ARG(batch >= 0, "negative batch");
+  UA_CHECK_ARG(batch == 0 || out != state, "1-qubit layers are out of place");
   UA_CHECK_ARG(num_targets >= 0 && num_targets <= n, "num_targets=%d out of range",
                num_targets);
   UA_CHECK_ARG(op_bstride >= 0 && target_stride >= 0, "negative operator stride");

@@ -142,7 +142,7 @@ int common_checks(void* out, const void* in, int dtype, int n, int64_t batch,
   UA_CHECK_ARG(dtype >= UA_FLOAT32 && dtype <= UA_COMPLEX128, "bad dtype %d", dtype);
   UA_CHECK_ARG(batch >= 0 && in_bstride >= 0, "negative batch or stride");
   UA_CHECK_ARG(batch == 0 || (out && in), "NULL pointer");
-  UA_CHECK_ARG(out != in, "out must not alias the input");
+  UA_CHECK_ARG(batch == 0 || out != in, "out must not alias the input");
   return UA_OK;
 }
 

@@ -68,3 +68,42 @@ def test_validation_errors_before_launch(lib):
     rc = lib.ua_apply_operator(None, None, None, _lib.UA_COMPLEX64, 4, 1,
                                _lib.int_array([1]), 0, 16, 0, None)
     assert rc == _lib.UA_OK
+
+
+def test_validation_errors_of_layer_permutation_sampling_and_constructors(lib):
+    from unitair_amd import _lib
+    bad = _lib.UA_ERR_INVALID_ARGUMENT
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)   # never dereferenced
+    c64 = _lib.UA_COMPLEX64
+    # 1-qubit layer: repeated qubit, qubit out of range, bad operator stride, aliasing
+    assert lib.ua_apply_1q_layer(a, b, a, c64, 12, 2, _lib.int_array([3, 3]), 1, 4096, 0,
+                                 None) == bad
+    assert lib.ua_apply_1q_layer(a, b, a, c64, 12, 1, _lib.int_array([12]), 1, 4096, 0,
+                                 None) == bad
+    assert lib.ua_apply_1q_layer(a, b, a, c64, 12, 2, _lib.int_array([1, 2]), 1, 4096, 5,
+                                 None) == bad
+    assert lib.ua_apply_1q_layer(a, a, b, c64, 12, 1, _lib.int_array([1]), 1, 4096, 0,
+                                 None) == bad
+    assert lib.ua_apply_all_qubits(a, b, a, c64, 12, 1, 4096, 3, None) == bad
+    # permutation: not a permutation, aliasing
+    assert lib.ua_permute_qubits(a, b, c64, 4, _lib.int_array([0, 1, 1, 3]), 1, 16, None) == bad
+    assert "not a permutation" in lib.ua_last_error().decode()
+    assert lib.ua_permute_qubits(a, a, c64, 4, _lib.int_array([0, 1, 2, 3]), 1, 16, None) == bad
+    # sign gates: pair with equal bits, mask beyond the state
+    assert lib.ua_apply_sign_gates(a, b, c64, 4, 1, _lib.int_array([2, 2]), 0, 0, 1, 16,
+                                   None) == bad
+    assert lib.ua_apply_sign_gates(a, b, c64, 4, 0, None, 1, 1 << 4, 1, 16, None) == bad
+    # controlled flip: target among the controls, target out of range
+    assert lib.ua_apply_controlled_flip(a, b, c64, 4, 0b0101, 2, 1, 16, None) == bad
+    assert lib.ua_apply_controlled_flip(a, b, c64, 4, 0, 4, 1, 16, None) == bad
+    # sampling and constructors
+    assert lib.ua_sample(None, a, c64, 4, -1, 0, 0, b, None) == bad
+    assert lib.ua_sample(None, a, c64, 41, 5, 0, 0, b, None) == bad
+    assert lib.ua_sample_workspace_bytes(41) == -1
+    assert lib.ua_sample_workspace_bytes(12) == 8 * (4096 + 2)
+    assert lib.ua_rand_state(a, c64, 1, 1, 0, b, None) == bad
+    assert lib.ua_fill_state(a, c64, 4, 1, 1, 16, None) == bad
+    assert lib.ua_fill_state(a, c64, 4, 1, 2, 0, None) == bad
+    # zero-sized work succeeds without touching memory
+    assert lib.ua_apply_1q_layer(None, None, None, c64, 12, 0, None, 0, 4096, 0, None) == _lib.UA_OK
+    assert lib.ua_sample(None, None, c64, 4, 0, 0, 0, None, None) == _lib.UA_OK
