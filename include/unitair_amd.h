@@ -348,7 +348,10 @@ typedef struct {
 
 /* Run one fused pass on `batch` contiguous states.  `ops` is the device copy
  * of the pass's ops; `host_ops` (may be NULL) is the same array in host
- * memory, used only to validate it before launch. */
+ * memory: it is validated before launch, and when it holds no PHASE and no
+ * DENSE2 op a leaner kernel without those two op kinds runs the pass (host_ops
+ * must then be an exact copy of ops).  With host_ops NULL the full kernel
+ * runs. */
 int ua_run_fused_pass(void* state, int dtype, int num_qubits, int64_t batch,
                       const ua_fused_pass* pass, const ua_fused_op* ops,
                       const ua_fused_op* host_ops, ua_stream_t stream);

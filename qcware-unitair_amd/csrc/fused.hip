@@ -157,7 +157,12 @@ struct PassArgs {
   int stage_ops[UA_FUSED_MAX_STAGES + 1];
 };
 
-template <typename T>
+// Kernel variants by the op kinds a pass uses (register footprint, complex64):
+//   kFull   every kind (229 VGPRs, 2 waves per SIMD);
+//   kLight  no per-element phases and no 2-qubit matrices (164, 3 waves);
+//   kLayers only 1-qubit layer ops (4 waves).
+enum { kLayers = 0, kLight = 1, kFull = 2 };
+template <typename T, int VARIANT>
 __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
     T* state, const ua_fused_op* __restrict__ ops, PassArgs a) {
   using R = typename RealOf<T>::type;
@@ -264,18 +269,23 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
         const ua_fused_op op = ops[o];
         switch (op.kind) {
           case UA_FOP_DENSE1: {
+            if constexpr (VARIANT != kLayers) {
             const T* u = op.cache_slot >= 0 ? mcache + op.cache_slot
                                             : static_cast<const T*>(op.data) + b * op.data_bstride;
             dispatch_1q(c, op.a, u);
+            }
             break;
           }
           case UA_FOP_DENSE2: {
-            const T* u = op.cache_slot >= 0 ? mcache + op.cache_slot
-                                            : static_cast<const T*>(op.data) + b * op.data_bstride;
-            dispatch_2q(c, op.a, op.b, u);
+            if constexpr (VARIANT == kFull) {
+              const T* u = op.cache_slot >= 0 ? mcache + op.cache_slot
+                                              : static_cast<const T*>(op.data) + b * op.data_bstride;
+              dispatch_2q(c, op.a, op.b, u);
+            }
             break;
           }
           case UA_FOP_CNOT: {  // a: control state bit, b: target cube bit
+            if constexpr (VARIANT != kLayers) {
             const uint32_t ctrl = sel(op.a);
             switch (op.b) {
               case 0: cube_cnot<0>(c, ctrl); break;
@@ -283,9 +293,11 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
               case 2: cube_cnot<2>(c, ctrl); break;
               default: cube_cnot<3>(c, ctrl); break;
             }
+            }
             break;
           }
           case UA_FOP_CZ: {  // a, b: state bits
+            if constexpr (VARIANT != kLayers) {
             const uint32_t neg = sel(op.a) & sel(op.b);
 #pragma unroll
             for (int j = 0; j < kCube; ++j) {
@@ -294,18 +306,22 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
                 c[j].y = -c[j].y;
               }
             }
+            }
             break;
           }
           case UA_FOP_DIAG1: {  // a: state bit; data: (d0, d1)
+            if constexpr (VARIANT != kLayers) {
             const T* d = op.cache_slot >= 0 ? mcache + op.cache_slot
                                             : static_cast<const T*>(op.data) + b * op.data_bstride;
             const T d0 = d[0], d1 = d[1];
             const uint32_t one = sel(op.a);
 #pragma unroll
             for (int j = 0; j < kCube; ++j) c[j] = cmul(((one >> j) & 1) ? d1 : d0, c[j]);
+            }
             break;
           }
           case UA_FOP_CZ_LAYER: {  // data: one uint64 mask per state
+            if constexpr (VARIANT != kLayers) {
             // Sign of entry j: parity of popcount(g & (g >> 1) & mask) with
             // g = gbase + cube part x (disjoint bits).  Over GF(2) it splits
             // into P(gbase), a term linear in the cube bits (a cube bit at p
@@ -333,6 +349,7 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
                 c[j].y = -c[j].y;
               }
             }
+            }
             break;
           }
           case UA_FOP_LAYER1Q: {  // a: four 8-bit counts; DENSE1 sub-ops follow
@@ -355,9 +372,11 @@ __global__ __launch_bounds__(kThreads) void fused_pass_kernel(
             break;
           }
           case UA_FOP_PHASE: {  // data: 2^n real angles per state
-            const R* th = static_cast<const R*>(op.data) + b * op.data_bstride;
+            if constexpr (VARIANT == kFull) {
+              const R* th = static_cast<const R*>(op.data) + b * op.data_bstride;
 #pragma unroll
-            for (int j = 0; j < kCube; ++j) c[j] = cmul(expmi(th[gidx(j)]), c[j]);
+              for (int j = 0; j < kCube; ++j) c[j] = cmul(expmi(th[gidx(j)]), c[j]);
+            }
             break;
           }
           default:
@@ -483,12 +502,35 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
   const int64_t tiles_per_block = int64_t(kTile) >> m;
   const int64_t blocks = (a.total_tiles + tiles_per_block - 1) / tiles_per_block;
   UA_CHECK_ARG(blocks <= 0x7fffffffLL, "too many tiles");
-  if (dtype == UA_COMPLEX64)
-    fused_pass_kernel<float2><<<unsigned(blocks), kThreads, 0, s>>>(
-        static_cast<float2*>(state), ops, a);
-  else
-    fused_pass_kernel<double2><<<unsigned(blocks), kThreads, 0, s>>>(
-        static_cast<double2*>(state), ops, a);
+  // the variant follows the host op list; without it, the full kernel
+  int variant = kFull;
+  if (host_ops) {
+    bool heavy = false, only_layers = true;
+    for (int o = 0; o < pass->stage_ops[pass->num_stages]; ++o) {
+      const int kind = host_ops[o].kind;
+      heavy = heavy || kind == UA_FOP_PHASE || kind == UA_FOP_DENSE2;
+      if (kind == UA_FOP_LAYER1Q) {
+        const uint32_t counts = uint32_t(host_ops[o].a);
+        o += int((counts & 255u) + ((counts >> 8) & 255u) + ((counts >> 16) & 255u) +
+                 ((counts >> 24) & 255u));   // skip its sub-ops (validated above)
+      } else {
+        only_layers = false;
+      }
+    }
+    variant = only_layers ? kLayers : (heavy ? kFull : kLight);
+  }
+#define UA_FUSED_LAUNCH(T, V)                                                     \
+  fused_pass_kernel<T, V><<<unsigned(blocks), kThreads, 0, s>>>(static_cast<T*>(state), ops, a)
+  if (dtype == UA_COMPLEX64) {
+    if (variant == kLayers) UA_FUSED_LAUNCH(float2, kLayers);
+    else if (variant == kLight) UA_FUSED_LAUNCH(float2, kLight);
+    else UA_FUSED_LAUNCH(float2, kFull);
+  } else {
+    if (variant == kLayers) UA_FUSED_LAUNCH(double2, kLayers);
+    else if (variant == kLight) UA_FUSED_LAUNCH(double2, kLight);
+    else UA_FUSED_LAUNCH(double2, kFull);
+  }
+#undef UA_FUSED_LAUNCH
   return check_launch("fused_pass_kernel");
 }
 

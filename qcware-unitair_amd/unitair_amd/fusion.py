@@ -126,14 +126,7 @@ def _segment_stage(ops: List[_Item], cube_state_bits) -> list:
         out.append(("op", it))
     if run is not None:
         out.append(("layer", run))
-    # a run that is a single gate is cheaper as a plain op
-    flat = []
-    for kind, x in out:
-        if kind == "layer" and sum(len(l) for l in x) == 1:
-            flat.append(("op", next(g for l in x for g in l)))
-        else:
-            flat.append((kind, x))
-    return flat
+    return out
 
 
 def _items_from_circuit(circ: Circuit) -> List[_Item]:

@@ -134,9 +134,9 @@ def test_stage_segments_keep_the_stage_result():
 
     seg = _segment_stage(items, cube)
     kinds = [k for k, _ in seg]
-    # runs: {2,0,5,2} -> layer; cz; {3, diag on bit 1 (not a cube bit) ends the run};
-    # diag1 bit 1 stays an op; {0} alone -> plain op; dense2; {5} alone -> plain op
-    assert kinds == ["layer", "op", "op", "op", "op", "op", "op"]
+    # runs: {2,0,5,2} -> layer; cz; {3} (the diag on bit 1, not a cube bit, ends
+    # the run) -> layer; diag1 bit 1 stays an op; {0} -> layer; dense2; {5} -> layer
+    assert kinds == ["layer", "op", "layer", "op", "layer", "op", "layer"]
     layer = seg[0][1]
     assert [len(l) for l in layer] == [1, 1, 0, 1]   # bit 2 (cube pos 1) merged to one gate
     flat = []
