@@ -23,7 +23,9 @@ def rel(a, b):
 @pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
 @pytest.mark.parametrize("n,qubits", [(12, (0, 1, 2, 3, 4)), (12, (7, 8, 9, 10, 11)),
                                       (13, (11, 0, 6, 3, 9)), (14, (2, 13, 5, 8)),
-                                      (11, (10, 9, 8, 7)), (15, (4, 14, 0, 10, 7))])
+                                      (11, (10, 9, 8, 7)), (15, (4, 14, 0, 10, 7)),
+                                      # every target on index bit >= 6: the direct kernel
+                                      (13, (0, 3, 5, 6)), (16, (9, 1, 4, 0, 7))])
 def test_mfma_matches_oracle(device, dt, n, qubits):
     rng = np.random.default_rng(n + len(qubits))
     k = len(qubits)
@@ -37,17 +39,19 @@ def test_mfma_matches_oracle(device, dt, n, qubits):
     assert rel(out, ref) < TOL[dt]
 
 
-def test_mfma_equals_vector_path_closely(device):
+@pytest.mark.parametrize("qubits", [(15, 3, 9, 0, 12), (0, 2, 4, 6, 8), (3, 0, 8, 4)])
+def test_mfma_equals_vector_path_closely(device, qubits):
     lib = _lib.load()
     rng = np.random.default_rng(1)
     n = 16
-    op = torch.from_numpy(rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32)))
+    D = 2 ** len(qubits)
+    op = torch.from_numpy(rng.standard_normal((D, D)) + 1j * rng.standard_normal((D, D)))
     st = torch.from_numpy(seeded_state_np(n, (2,), seed=2))
     for dt in (torch.complex64, torch.complex128):
-        a = sim.apply_operator(op.to(device, dt), (15, 3, 9, 0, 12), st.to(device, dt))
+        a = sim.apply_operator(op.to(device, dt), qubits, st.to(device, dt))
         lib.ua_set_tuning(_lib.UA_TUNE_MFMA, 0)
         try:
-            b = sim.apply_operator(op.to(device, dt), (15, 3, 9, 0, 12), st.to(device, dt))
+            b = sim.apply_operator(op.to(device, dt), qubits, st.to(device, dt))
         finally:
             lib.ua_set_tuning(_lib.UA_TUNE_MFMA, 1)
         assert rel(a.cpu().numpy(), b.cpu().numpy()) < TOL[dt]
