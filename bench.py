@@ -328,7 +328,18 @@ def _ops_inputs(n, seed, device, dtype):
     return psi, other, u, ang, diag
 
 
-def _op_table(n, sim, states, psi, other, u, ang, diag, sample):
+def _grad_call(apply_operator, u, psi):
+    """Forward + backward of one 1-qubit apply_operator with respect to the
+    state and the operator (the drop-in's autograd)."""
+    import torch
+    x = psi.detach().requires_grad_(True)
+    op = u.detach().requires_grad_(True)
+    y = apply_operator(op, [3], x)
+    y.real.sum().backward()
+    return x.grad
+
+
+def _op_table(n, sim, states, psi, other, u, ang, diag, sample, rand_state):
     """(name, callable, algorithmic bytes) for each drop-in call, complex64."""
     import torch
     A, s = 2 ** n, 8
@@ -357,6 +368,9 @@ def _op_table(n, sim, states, psi, other, u, ang, diag, sample):
         ("multi_cz, ring of n pairs", lambda: sim.multi_cz(ring, psi), 2 * A * s),
         ("measure: draw 2^20 samples (probabilities, prefix sums, draws)",
          lambda: sample(psi, 1 << 20), A * s + A * 8),
+        ("rand_state (Gaussian, normalised per state)", lambda: rand_state(n), A * s),
+        ("apply_operator 1q forward + backward (autograd, state and operator)",
+         lambda: _grad_call(sim.apply_operator, u[1], psi), 6 * A * s),
         ("norm_squared", lambda: states.norm_squared(psi), A * s),
         ("abs_squared", lambda: states.abs_squared(psi), A * s + A * 4),
         ("inner_product", lambda: states.inner_product(psi, other), 2 * A * s),
@@ -375,8 +389,10 @@ def ops_bench(dev, n=24, seed=5, reps=20):
     psi, other, u, ang, diag = _ops_inputs(n, seed, dev, torch.complex64)
     out = {}
     from unitair_amd.measurement import sample_indices
+    from unitair_amd.initializations import rand_state
     for name, fn, nbytes in _op_table(n, sim, ua.states, psi, other, u, ang, diag,
-                                      sample_indices):
+                                      sample_indices,
+                                      lambda k: rand_state(k, device=dev)):
         fn()
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -419,8 +435,12 @@ def cpu_ops_baseline(n=24, seed=5, budget_s=10.0):
 
     out = {}
     t_start = time.perf_counter()
+    def cpu_rand_state(k):
+        x = torch.randn(2 ** k, dtype=torch.complex64)
+        return x / x.abs().square().sum().sqrt()
+
     for name, fn, nbytes in _op_table(n, _Sim, _States, psi, other, u, ang, diag,
-                                      T.sample_indices):
+                                      T.sample_indices, cpu_rand_state):
         t0 = time.perf_counter()
         fn()
         us = (time.perf_counter() - t0) * 1e6
