@@ -506,7 +506,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this "
-              f"build (the state-vector path is single-GPU, see DESIGN.md)", file=sys.stderr)
+              f"build: it measures the state-vector kernels on one GPU", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
