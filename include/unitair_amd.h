@@ -366,13 +366,18 @@ int ua_run_fused_pass(void* state, int dtype, int num_qubits, int64_t batch,
  * op_count is 1 (one operator for the whole batch) or batch.  grad_out is
  * contiguous (batch, 2^n); state rows start at state + b*state_batch_stride.
  * k <= UA_MAX_GRAD_QUBITS.  The gradient w.r.t. the state is
- * ua_apply_operator with the conjugate-transposed operator.
+ * ua_apply_operator with the conjugate-transposed operator.  The reduction
+ * over groups is split across many workgroups; the float64 partial sums go to
+ * `workspace` (ua_operator_gradient_workspace_bytes, 8-byte aligned; 0 bytes,
+ * and NULL allowed, when one workgroup per result block suffices).
  */
 #define UA_MAX_GRAD_QUBITS 16
+int64_t ua_operator_gradient_workspace_bytes(int num_qubits, int op_num_qubits, int64_t batch,
+                                             int64_t op_count);
 int ua_operator_gradient(void* grad_op, const void* grad_out, const void* state, int dtype,
                          int num_qubits, int op_num_qubits, const int* qubits,
                          int64_t batch, int64_t op_count, int64_t state_batch_stride,
-                         ua_stream_t stream);
+                         void* workspace, ua_stream_t stream);
 
 /*
  * Adjoint differentiation of Pauli-rotation circuits for H = sum_i Z_i (the

@@ -356,7 +356,18 @@ extern "C" int ua_expect_sum_z(double* out, const void* psi, int dtype, int num_
 // unitair/simulation/operations.py:322, out = U @ in on the target bits):
 //   grad_U[o][r][c] = sum over batch entries b using operator o, and over
 //                     groups g:  grad_out[b][g|off(r)] * conj(in[b][g|off(c)])
-// One workgroup per (o, r, c); float64 accumulation.
+// i.e. grad_U = G^T conj(X) with G, X the (groups x 2^k) views of grad_out
+// and in: a tall-skinny product whose reduction runs over 2^(n-k) groups.
+//
+// op_grad_partial_kernel: a workgroup owns one operator o, one slice ("part")
+// of that operator's groups and one RB x CB block of the 2^k x 2^k result.
+// Each thread walks the slice with stride 256 (consecutive threads take
+// consecutive groups, so every load is coalesced), keeps the block's RB*CB
+// products in float64 registers, and the workgroup reduces them with
+// wavefront shuffles and LDS.  With one part the sums go straight to grad_op;
+// otherwise to the float64 workspace, and op_grad_finish_kernel adds the
+// parts (one workgroup per result entry).  k = 1, 2 take the whole result in one block (each amplitude read
+// once); larger k tile it 4 x 4.
 namespace ua {
 namespace {
 
@@ -364,69 +375,198 @@ struct OpGradParams {
   int n, k;
   int64_t batch, op_count;   // op_count: distinct operators (1 or batch)
   int64_t in_bstride;        // 0 when one input state serves the whole batch
-  int tbit[UA_MAX_GRAD_QUBITS];  // state bit of gate-index bit k-1-m (qubits order)
+  int64_t parts;             // slices of each operator's groups
+  int64_t units;             // groups per operator: (batch / op_count) << (n - k)
+  int64_t units_per_part;
+  int nblk_c;                // column blocks: 2^k / CB
+  int tbit[UA_MAX_GRAD_QUBITS];    // state bit of gate-index bit k-1-m (qubits order)
+  int sorted[UA_MAX_GRAD_QUBITS];  // target state bits, ascending
 };
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void op_grad_kernel(
-    T* __restrict__ grad_op, const T* __restrict__ gout, const T* __restrict__ in,
-    OpGradParams p) {
-  __shared__ double scratch[kBlock / 64];
+template <typename T> __device__ __forceinline__ double2 as_d2(T v);
+template <> __device__ __forceinline__ double2 as_d2(float v) { return make_double2(v, 0.0); }
+template <> __device__ __forceinline__ double2 as_d2(double v) { return make_double2(v, 0.0); }
+template <> __device__ __forceinline__ double2 as_d2(float2 v) { return make_double2(v.x, v.y); }
+template <> __device__ __forceinline__ double2 as_d2(double2 v) { return v; }
+
+template <typename T> __device__ __forceinline__ T from_d2(double2 v);
+template <> __device__ __forceinline__ float from_d2(double2 v) { return float(v.x); }
+template <> __device__ __forceinline__ double from_d2(double2 v) { return v.x; }
+template <> __device__ __forceinline__ float2 from_d2(double2 v) { return make_float2(float(v.x), float(v.y)); }
+template <> __device__ __forceinline__ double2 from_d2(double2 v) { return v; }
+
+__device__ __forceinline__ uint64_t gate_offset(int idx, const OpGradParams& p) {
+  uint64_t o = 0;
+  for (int m = 0; m < p.k; ++m)
+    if ((idx >> (p.k - 1 - m)) & 1) o |= uint64_t(1) << p.tbit[m];
+  return o;
+}
+
+template <typename T, int RB, int CB>
+__global__ __launch_bounds__(kBlock) void op_grad_partial_kernel(
+    T* __restrict__ grad_op, double* __restrict__ part_out, const T* __restrict__ gout,
+    const T* __restrict__ in, OpGradParams p) {
+  __shared__ double2 red[kBlock / 64][RB * CB];
   const int64_t D = int64_t(1) << p.k;
-  const int64_t o = blockIdx.x / (D * D);
-  const int64_t r = (blockIdx.x / D) % D;
-  const int64_t c = blockIdx.x % D;
-  uint64_t offr = 0, offc = 0, mask = 0;
-  for (int m = 0; m < p.k; ++m) {
-    const uint64_t bit = uint64_t(1) << p.tbit[m];
-    mask |= bit;
-    if ((r >> (p.k - 1 - m)) & 1) offr |= bit;
-    if ((c >> (p.k - 1 - m)) & 1) offc |= bit;
-  }
+  const int blk = int(blockIdx.x % unsigned(p.nblk_c * (D / RB)));
+  const int64_t rest = int64_t(blockIdx.x) / (p.nblk_c * (D / RB));
+  const int64_t part = rest % p.parts;
+  const int64_t o = rest / p.parts;
+  const int r0 = (blk / p.nblk_c) * RB, c0 = (blk % p.nblk_c) * CB;
+  uint64_t offr[RB], offc[CB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) offr[i] = gate_offset(r0 + i, p);
+#pragma unroll
+  for (int j = 0; j < CB; ++j) offc[j] = gate_offset(c0 + j, p);
+
+  const int gbits = p.n - p.k;
   const int64_t N = int64_t(1) << p.n;
-  const int64_t groups = N >> p.k;
-  const int64_t per_op = p.batch / p.op_count;   // batch entries sharing operator o
-  double sr = 0.0, si = 0.0;
-  for (int64_t t = threadIdx.x; t < per_op * groups; t += kBlock) {
-    const int64_t b = (p.op_count == 1) ? t / groups : o;
-    const int64_t gi = t % groups;
-    // insert zeros at the target bits (ascending order of bit position)
-    uint64_t base = uint64_t(gi);
-    for (int bitpos = 0; bitpos < p.n; ++bitpos)
-      if ((mask >> bitpos) & 1) {
-        const uint64_t low = base & ((uint64_t(1) << bitpos) - 1);
-        base = ((base >> bitpos) << (bitpos + 1)) | low;
+  const int64_t u_begin = part * p.units_per_part;
+  int64_t u_end = u_begin + p.units_per_part;
+  if (u_end > p.units) u_end = p.units;
+  double2 acc[RB][CB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < CB; ++j) acc[i][j] = make_double2(0.0, 0.0);
+  for (int64_t u = u_begin + threadIdx.x; u < u_end; u += kBlock) {
+    const int64_t b = (p.op_count == 1) ? (u >> gbits) : o;
+    const uint64_t gi = uint64_t(u) & ((uint64_t(1) << gbits) - 1);
+    const uint64_t base = insert_zero_bits(gi, p.sorted, p.k);
+    const T* gb = gout + b * N + base;
+    const T* xb = in + b * p.in_bstride + base;
+    double2 g[RB], x[CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) g[i] = as_d2(gb[offr[i]]);
+#pragma unroll
+    for (int j = 0; j < CB; ++j) x[j] = as_d2(xb[offc[j]]);
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int j = 0; j < CB; ++j) {
+        // g * conj(x)
+        acc[i][j].x = fma(g[i].x, x[j].x, fma(g[i].y, x[j].y, acc[i][j].x));
+        acc[i][j].y = fma(g[i].y, x[j].x, fma(-g[i].x, x[j].y, acc[i][j].y));
       }
-    if constexpr (Num<T>::complex) {
-      const T g = gout[b * N + (base | offr)];
-      const T x = in[b * p.in_bstride + (base | offc)];
-      sr += double(g.x) * x.x + double(g.y) * x.y;    // g * conj(x)
-      si += double(g.y) * x.x - double(g.x) * x.y;
+  }
+  // workgroup reduction: shuffles inside each wavefront, then LDS
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      double vr = acc[i][j].x, vi = acc[i][j].y;
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) {
+        vr += __shfl_xor(vr, d, 64);
+        vi += __shfl_xor(vi, d, 64);
+      }
+      if (lane == 0) red[wave][i * CB + j] = make_double2(vr, vi);
+    }
+  __syncthreads();
+  if (threadIdx.x < RB * CB) {
+    double2 t = red[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) {
+      t.x += red[w][threadIdx.x].x;
+      t.y += red[w][threadIdx.x].y;
+    }
+    const int64_t e = int64_t(r0 + threadIdx.x / CB) * D + (c0 + threadIdx.x % CB);
+    if (p.parts == 1) {
+      grad_op[o * D * D + e] = from_d2<T>(t);
     } else {
-      sr += double(gout[b * N + (base | offr)]) * in[b * p.in_bstride + (base | offc)];
+      double* dst = part_out + ((o * p.parts + part) * D * D + e) * 2;
+      dst[0] = t.x;
+      dst[1] = t.y;
     }
   }
-  const double tr = block_sum(sr, scratch);
-  const double ti = block_sum(si, scratch);
+}
+
+// One workgroup per (operator, entry): the threads split the parts, then
+// reduce with wavefront shuffles and LDS.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void op_grad_finish_kernel(T* __restrict__ grad_op,
+                                                                const double* __restrict__ part,
+                                                                int64_t parts, int64_t DD) {
+  __shared__ double2 red[kBlock / 64];
+  const int64_t t = blockIdx.x;  // o * DD + e
+  const int64_t o = t / DD, e = t % DD;
+  double vr = 0.0, vi = 0.0;
+  for (int64_t q = threadIdx.x; q < parts; q += kBlock) {
+    const double* src = part + ((o * parts + q) * DD + e) * 2;
+    vr += src[0];
+    vi += src[1];
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    vr += __shfl_xor(vr, d, 64);
+    vi += __shfl_xor(vi, d, 64);
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double2(vr, vi);
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if constexpr (Num<T>::complex) {
-      T v;
-      v.x = tr;
-      v.y = ti;
-      grad_op[blockIdx.x] = v;
-    } else {
-      grad_op[blockIdx.x] = T(tr);
+    double2 sum = red[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) {
+      sum.x += red[w].x;
+      sum.y += red[w].y;
     }
+    grad_op[t] = from_d2<T>(sum);
   }
+}
+
+// Launch shape shared by the workspace query and the entry point.
+struct OpGradShape {
+  int rb, cb;
+  int64_t nblk, parts, units, units_per_part;
+};
+
+OpGradShape op_grad_shape(int n, int k, int64_t batch, int64_t op_count) {
+  OpGradShape sh{};
+  const int64_t D = int64_t(1) << k;
+  sh.rb = sh.cb = int(D < 4 ? D : 4);
+  sh.nblk = (D / sh.rb) * (D / sh.cb);
+  sh.units = (batch / op_count) << (n - k);
+  // about 4096 workgroups in all, each thread with at least 16 groups
+  const int64_t want = (4096 + op_count * sh.nblk - 1) / (op_count * sh.nblk);
+  const int64_t most = (sh.units + kBlock * 16 - 1) / (kBlock * 16);
+  sh.parts = want < most ? want : most;
+  if (sh.parts < 1) sh.parts = 1;
+  sh.units_per_part = (sh.units + sh.parts - 1) / sh.parts;
+  return sh;
+}
+
+template <typename T>
+void launch_op_grad(int rb, unsigned grid, T* grad, double* part, const T* g, const T* x,
+                    const OpGradParams& p, hipStream_t s) {
+  if (rb == 1)
+    op_grad_partial_kernel<T, 1, 1><<<grid, kBlock, 0, s>>>(grad, part, g, x, p);
+  else if (rb == 2)
+    op_grad_partial_kernel<T, 2, 2><<<grid, kBlock, 0, s>>>(grad, part, g, x, p);
+  else
+    op_grad_partial_kernel<T, 4, 4><<<grid, kBlock, 0, s>>>(grad, part, g, x, p);
 }
 
 }  // namespace
 }  // namespace ua
 
+extern "C" int64_t ua_operator_gradient_workspace_bytes(int num_qubits, int op_num_qubits,
+                                                       int64_t batch, int64_t op_count) {
+  const int n = num_qubits, k = op_num_qubits;
+  if (n < 1 || n > 62 || k < 0 || k > n || k > UA_MAX_GRAD_QUBITS || batch < 0 ||
+      !(op_count == 1 || op_count == batch))
+    return -1;
+  if (batch == 0) return 0;
+  const ua::OpGradShape sh = ua::op_grad_shape(n, k, batch, op_count);
+  if (sh.parts == 1) return 0;
+  return op_count * sh.parts * (int64_t(1) << (2 * k)) * 2 * int64_t(sizeof(double));
+}
+
 extern "C" int ua_operator_gradient(void* grad_op, const void* grad_out, const void* state,
                                     int dtype, int num_qubits, int op_num_qubits,
                                     const int* qubits, int64_t batch, int64_t op_count,
-                                    int64_t state_batch_stride, ua_stream_t stream) {
+                                    int64_t state_batch_stride, void* workspace,
+                                    ua_stream_t stream) {
   using namespace ua;
   const int n = num_qubits, k = op_num_qubits;
   UA_CHECK_ARG(elem_size(dtype) != 0, "bad dtype");
@@ -448,14 +588,36 @@ extern "C" int ua_operator_gradient(void* grad_op, const void* grad_out, const v
     UA_CHECK_ARG(!((seen >> p.tbit[m]) & 1), "duplicate qubit");
     seen |= uint64_t(1) << p.tbit[m];
   }
-  const int64_t blocks = op_count * (int64_t(1) << (2 * k));
+  for (int b = 0, m = 0; b < n; ++b)
+    if ((seen >> b) & 1) p.sorted[m++] = b;
+  const OpGradShape sh = op_grad_shape(n, k, batch, op_count);
+  p.parts = sh.parts;
+  p.units = sh.units;
+  p.units_per_part = sh.units_per_part;
+  p.nblk_c = int((int64_t(1) << k) / sh.cb);
+  const int64_t blocks = op_count * sh.parts * sh.nblk;
   UA_CHECK_ARG(blocks <= 0x7fffffffLL, "grid too large");
+  UA_CHECK_ARG(sh.parts == 1 || (workspace && reinterpret_cast<uintptr_t>(workspace) % 8 == 0),
+               "workspace (ua_operator_gradient_workspace_bytes) missing or misaligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(workspace);
+  const unsigned grid = unsigned(blocks);
   switch (dtype) {
-    case UA_FLOAT32: op_grad_kernel<float><<<unsigned(blocks), kBlock, 0, s>>>((float*)grad_op, (const float*)grad_out, (const float*)state, p); break;
-    case UA_FLOAT64: op_grad_kernel<double><<<unsigned(blocks), kBlock, 0, s>>>((double*)grad_op, (const double*)grad_out, (const double*)state, p); break;
-    case UA_COMPLEX64: op_grad_kernel<float2><<<unsigned(blocks), kBlock, 0, s>>>((float2*)grad_op, (const float2*)grad_out, (const float2*)state, p); break;
-    case UA_COMPLEX128: op_grad_kernel<double2><<<unsigned(blocks), kBlock, 0, s>>>((double2*)grad_op, (const double2*)grad_out, (const double2*)state, p); break;
+    case UA_FLOAT32: launch_op_grad(sh.rb, grid, (float*)grad_op, part, (const float*)grad_out, (const float*)state, p, s); break;
+    case UA_FLOAT64: launch_op_grad(sh.rb, grid, (double*)grad_op, part, (const double*)grad_out, (const double*)state, p, s); break;
+    case UA_COMPLEX64: launch_op_grad(sh.rb, grid, (float2*)grad_op, part, (const float2*)grad_out, (const float2*)state, p, s); break;
+    case UA_COMPLEX128: launch_op_grad(sh.rb, grid, (double2*)grad_op, part, (const double2*)grad_out, (const double2*)state, p, s); break;
   }
-  return check_launch("op_grad_kernel");
+  if (int rc = check_launch("op_grad_partial_kernel")) return rc;
+  if (sh.parts == 1) return UA_OK;
+  const int64_t DD = int64_t(1) << (2 * k);
+  UA_CHECK_ARG(op_count * DD <= 0x7fffffffLL, "grid too large");
+  const unsigned fgrid = unsigned(op_count * DD);
+  switch (dtype) {
+    case UA_FLOAT32: op_grad_finish_kernel<float><<<fgrid, kBlock, 0, s>>>((float*)grad_op, part, sh.parts, DD); break;
+    case UA_FLOAT64: op_grad_finish_kernel<double><<<fgrid, kBlock, 0, s>>>((double*)grad_op, part, sh.parts, DD); break;
+    case UA_COMPLEX64: op_grad_finish_kernel<float2><<<fgrid, kBlock, 0, s>>>((float2*)grad_op, part, sh.parts, DD); break;
+    case UA_COMPLEX128: op_grad_finish_kernel<double2><<<fgrid, kBlock, 0, s>>>((double2*)grad_op, part, sh.parts, DD); break;
+  }
+  return check_launch("op_grad_finish_kernel");
 }

@@ -4,10 +4,12 @@
 // (initializations.py:71-100: torch.randn, then divide by the norm).  The
 // normals come from Philox4x32-10 (Salmon et al., SC'11) with counter
 // (element pair, 0, 0, 0) and key (seed), through the Box-Muller transform
-// (Box and Muller, 1958).  Because they are counter-based, the first pass
-// generates them only to sum |x|^2 in float64 per block, and the last pass
-// generates the same numbers again and writes them scaled: the state is
-// written once and never read back.
+// (Box and Muller, 1958), in float64 for float64/complex128 states and in
+// float32 (to a few ulp of the float64 transform) for float32/complex64.
+// The generation pass writes the normals unscaled and sums |x|^2 in float64
+// per block; after the per-state norms, a streaming pass scales in place.
+// (Generating is compute-bound, so one generation plus a read-modify-write
+// pass beats generating twice.)
 // ua_fill_state: uniform_superposition (2^(-n/2) everywhere, :103-128) and
 // unit_vector (one 1, :46-68).
 #include "ua_common.h"
@@ -35,8 +37,13 @@ __device__ __forceinline__ void philox(uint32_t (&c)[4], uint32_t k0, uint32_t k
   }
 }
 
-// Four standard normals for counter `ctr`.
-__device__ __forceinline__ void normals4(uint64_t ctr, uint64_t seed, double (&z)[4]) {
+// Four standard normals for counter `ctr`, in the arithmetic of R (float64
+// for float64/complex128 states, float32 for float32/complex64 ones).
+template <typename R>
+__device__ __forceinline__ void normals4(uint64_t ctr, uint64_t seed, R (&z)[4]);
+
+template <>
+__device__ __forceinline__ void normals4<double>(uint64_t ctr, uint64_t seed, double (&z)[4]) {
   uint32_t c[4] = {uint32_t(ctr), uint32_t(ctr >> 32), 0u, 0u};
   philox(c, uint32_t(seed), uint32_t(seed >> 32));
 #pragma unroll
@@ -51,29 +58,63 @@ __device__ __forceinline__ void normals4(uint64_t ctr, uint64_t seed, double (&z
   }
 }
 
+// The same transform in float32.  u1 itself is not representable to 32 bits
+// in float32, so ln(u1) is taken from whichever of u1 and 1 - u1 is exact to
+// float32 rounding: ln(u1) for u1 <= 1/2, log1p(-(1 - u1)) above (1 - u1 =
+// (2^32 - 1 - x) 2^-32 is an exact integer times 2^-32).  Every normal then
+// carries a float32 relative error of a few ulp of the float64 transform.
+template <>
+__device__ __forceinline__ void normals4<float>(uint64_t ctr, uint64_t seed, float (&z)[4]) {
+  uint32_t c[4] = {uint32_t(ctr), uint32_t(ctr >> 32), 0u, 0u};
+  philox(c, uint32_t(seed), uint32_t(seed >> 32));
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t x = c[2 * h];
+    const float l = (x < 0x80000000u)
+                        ? logf((float(x) + 1.0f) * 0x1.0p-32f)
+                        : log1pf(-float(0xFFFFFFFFu - x) * 0x1.0p-32f);
+    const float r = sqrtf(-2.0f * l);
+    // angle 2 pi u2: the quadrant from the top two bits of y (exact), the
+    // rest (< pi/2) from the low 30 bits
+    const uint32_t y = c[2 * h + 1];
+    float s0, c0;
+    sincospif(float(y & 0x3FFFFFFFu) * 0x1.0p-31f, &s0, &c0);
+    const uint32_t quad = y >> 30;
+    const float sq0 = (quad & 1) ? c0 : s0, cq0 = (quad & 1) ? s0 : c0;
+    const float s = (quad >= 2) ? -sq0 : sq0;                 // sin(a + quad pi/2)
+    const float co = (quad == 1 || quad == 2) ? -cq0 : cq0;   // cos(a + quad pi/2)
+    z[2 * h] = r * co;
+    z[2 * h + 1] = r * s;
+  }
+}
+
 // Element e of the flat (batch * 2^n) array gets normals from counter
 // e / per (per = 4 elements per counter for real states, 2 for complex).
 template <typename T> struct Gen;
 template <> struct Gen<float> {
+  using R = float;
   static constexpr int per = 4;
-  __device__ static float make(const double* z, int j, double scale) { return float(z[j] * scale); }
-  __device__ static double sq(const double* z, int j) { return z[j] * z[j]; }
+  __device__ static float make(const float* z, int j, double scale) { return float(z[j] * scale); }
+  __device__ static double sq(const float* z, int j) { return double(z[j]) * z[j]; }
 };
 template <> struct Gen<double> {
+  using R = double;
   static constexpr int per = 4;
   __device__ static double make(const double* z, int j, double scale) { return z[j] * scale; }
   __device__ static double sq(const double* z, int j) { return z[j] * z[j]; }
 };
 template <> struct Gen<float2> {
+  using R = float;
   static constexpr int per = 2;
-  __device__ static float2 make(const double* z, int j, double scale) {
+  __device__ static float2 make(const float* z, int j, double scale) {
     return make_float2(float(z[2 * j] * scale), float(z[2 * j + 1] * scale));
   }
-  __device__ static double sq(const double* z, int j) {
-    return z[2 * j] * z[2 * j] + z[2 * j + 1] * z[2 * j + 1];
+  __device__ static double sq(const float* z, int j) {
+    return double(z[2 * j]) * z[2 * j] + double(z[2 * j + 1]) * z[2 * j + 1];
   }
 };
 template <> struct Gen<double2> {
+  using R = double;
   static constexpr int per = 2;
   __device__ static double2 make(const double* z, int j, double scale) {
     return make_double2(z[2 * j] * scale, z[2 * j + 1] * scale);
@@ -83,19 +124,23 @@ template <> struct Gen<double2> {
   }
 };
 
-// Pass 1: block k sums |x|^2 over counters [k * G, (k + 1) * G), G counters
-// per block; blocks never straddle two states.
+// Pass 1: block k generates counters [k * G, (k + 1) * G) (G counters per
+// block; blocks never straddle two states), writes the normals unscaled and
+// sums their |x|^2 in float64.
 template <typename T>
-__global__ __launch_bounds__(kInitThreads) void rand_norm_kernel(int64_t G, uint64_t seed,
-                                                                 double* partial) {
+__global__ __launch_bounds__(kInitThreads) void rand_gen_kernel(T* out, int64_t G, uint64_t seed,
+                                                                double* partial) {
   __shared__ double red[kInitThreads / 64];
   double acc = 0;
   const int64_t c0 = int64_t(blockIdx.x) * G;
-  for (int64_t c = threadIdx.x; c < G; c += kInitThreads) {
-    double z[4];
-    normals4(uint64_t(c0 + c), seed, z);
+  for (int64_t c = c0 + threadIdx.x; c < c0 + G; c += kInitThreads) {
+    typename Gen<T>::R z[4];
+    normals4(uint64_t(c), seed, z);
 #pragma unroll
-    for (int j = 0; j < Gen<T>::per; ++j) acc += Gen<T>::sq(z, j);
+    for (int j = 0; j < Gen<T>::per; ++j) {
+      acc += Gen<T>::sq(z, j);
+      out[c * Gen<T>::per + j] = Gen<T>::make(z, j, 1.0);
+    }
   }
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) acc += __shfl_xor(acc, d, 64);
@@ -127,20 +172,29 @@ __global__ __launch_bounds__(kInitThreads) void rand_scale_kernel(const double* 
   }
 }
 
-// Pass 2: the same normals again, scaled, written once.
+// Pass 2: scale in place, two elements per thread (states hold >= 4).  The
+// product is formed in float64 and rounded once, as if the normals had been
+// scaled before they were first stored.
+template <typename T> __device__ __forceinline__ T scaled(T v, double f);
+template <> __device__ __forceinline__ float scaled(float v, double f) { return float(v * f); }
+template <> __device__ __forceinline__ double scaled(double v, double f) { return v * f; }
+template <> __device__ __forceinline__ float2 scaled(float2 v, double f) {
+  return make_float2(float(v.x * f), float(v.y * f));
+}
+template <> __device__ __forceinline__ double2 scaled(double2 v, double f) {
+  return make_double2(v.x * f, v.y * f);
+}
+
 template <typename T>
-__global__ __launch_bounds__(kInitThreads) void rand_write_kernel(T* out, int64_t total_ctr,
-                                                                  int64_t ctr_per_state,
-                                                                  uint64_t seed,
+__global__ __launch_bounds__(kInitThreads) void rand_apply_kernel(T* out, int64_t pairs, int n,
                                                                   const double* inv_norm) {
   const int64_t stride = int64_t(gridDim.x) * kInitThreads;
-  for (int64_t c = int64_t(blockIdx.x) * kInitThreads + threadIdx.x; c < total_ctr;
-       c += stride) {
-    double z[4];
-    normals4(uint64_t(c), seed, z);
-    const double scale = inv_norm[c / ctr_per_state];
-#pragma unroll
-    for (int j = 0; j < Gen<T>::per; ++j) out[c * Gen<T>::per + j] = Gen<T>::make(z, j, scale);
+  for (int64_t q = int64_t(blockIdx.x) * kInitThreads + threadIdx.x; q < pairs; q += stride) {
+    const double f = inv_norm[(q * 2) >> n];
+    Pair<T> v = reinterpret_cast<Pair<T>*>(out)[q];
+    v.v[0] = scaled(v.v[0], f);
+    v.v[1] = scaled(v.v[1], f);
+    reinterpret_cast<Pair<T>*>(out)[q] = v;
   }
 }
 
@@ -166,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(T* out, int64_t total, int
 int64_t counters_per_block(int dtype, int n) {
   const int per = (dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128) ? 2 : 4;
   const int64_t ctr_per_state = (int64_t(1) << n) / per;
-  return ctr_per_state < 16384 ? ctr_per_state : 16384;
+  return ctr_per_state < 2048 ? ctr_per_state : 2048;
 }
 
 }  // namespace
@@ -198,17 +252,18 @@ extern "C" int ua_rand_state(void* out, int dtype, int num_qubits, int64_t batch
   double* partial = static_cast<double*>(workspace);
   double* inv_norm = partial + blocks;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int64_t total_ctr = batch * ctr_per_state;
-  const int64_t wb = (total_ctr + kInitThreads - 1) / kInitThreads;
-  const unsigned wgrid = unsigned(wb < 65536 ? wb : 65536);
-#define UA_RAND(T)                                                                   \
-  rand_norm_kernel<T><<<unsigned(blocks), kInitThreads, 0, s>>>(G, seed, partial);   \
-  if (int rc = check_launch("rand_norm_kernel")) return rc;                          \
+  const int64_t pairs = (batch << num_qubits) / 2;
+  const int64_t ab = (pairs + kInitThreads - 1) / kInitThreads;
+  const unsigned agrid = unsigned(ab < 65536 ? ab : 65536);
+#define UA_RAND(T)                                                                     \
+  rand_gen_kernel<T><<<unsigned(blocks), kInitThreads, 0, s>>>(static_cast<T*>(out), G, \
+                                                               seed, partial);         \
+  if (int rc = check_launch("rand_gen_kernel")) return rc;                             \
   rand_scale_kernel<<<unsigned(batch), kInitThreads, 0, s>>>(partial, blocks_per_state, \
-                                                             inv_norm);              \
-  if (int rc = check_launch("rand_scale_kernel")) return rc;                         \
-  rand_write_kernel<T><<<wgrid, kInitThreads, 0, s>>>(static_cast<T*>(out), total_ctr, \
-                                                      ctr_per_state, seed, inv_norm);
+                                                             inv_norm);                \
+  if (int rc = check_launch("rand_scale_kernel")) return rc;                           \
+  rand_apply_kernel<T><<<agrid, kInitThreads, 0, s>>>(static_cast<T*>(out), pairs,      \
+                                                      num_qubits, inv_norm);
   switch (dtype) {
     case UA_FLOAT32: UA_RAND(float); break;
     case UA_FLOAT64: UA_RAND(double); break;
@@ -216,7 +271,7 @@ extern "C" int ua_rand_state(void* out, int dtype, int num_qubits, int64_t batch
     default: UA_RAND(double2); break;
   }
 #undef UA_RAND
-  return check_launch("rand_write_kernel");
+  return check_launch("rand_apply_kernel");
 }
 
 extern "C" int ua_fill_state(void* out, int dtype, int num_qubits, int64_t batch, int kind,

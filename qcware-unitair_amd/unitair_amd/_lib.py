@@ -64,7 +64,9 @@ SIGNATURES = {
     "ua_inner_product": (_i32, [_p, _i32, _p, _i64, _p, _i64, _i32, _i64, _i64, _p, _p]),
     "ua_diag_expectation": (_i32, [_p, _i32, _p, _i32, _i64, _p, _i32, _i64, _i64, _i64, _p, _p]),
     "ua_abs_squared": (_i32, [_p, _p, _i32, _i64, _p]),
-    "ua_operator_gradient": (_i32, [_p, _p, _p, _i32, _i32, _i32, _pi32, _i64, _i64, _i64, _p]),
+    "ua_operator_gradient_workspace_bytes": (_i64, [_i32, _i32, _i64, _i64]),
+    "ua_operator_gradient": (_i32, [_p, _p, _p, _i32, _i32, _i32, _pi32, _i64, _i64, _i64, _p,
+                                    _p]),
     "ua_pauli_gradients_workspace": (_i64, [_i32, _i64]),
     "ua_expect_sum_z": (_i32, [_p, _p, _i32, _i32, _i64, _p, _p]),
     "ua_apply_sum_z": (_i32, [_p, _p, _i32, _i32, _i64, _p]),

@@ -187,11 +187,15 @@ def _operator_gradient(g, state, op, qubits, n, k, out_batch, expand_state, op_b
                                   f"{_lib.UA_MAX_GRAD_QUBITS} qubits")
     s_flat, s_stride = (state.reshape(N), 0) if expand_state else _flat_batch(state, N)
     op_count = 1 if op_bstride == 0 else batch
+    # a negative size flags bad arguments, which the entry point then reports
+    nbytes = lib.ua_operator_gradient_workspace_bytes(n, k, batch, op_count)
+    work = torch.empty(nbytes, dtype=torch.uint8, device=state.device) if nbytes > 0 else None
     with on_device(state.device):
         rc = lib.ua_operator_gradient(
             _lib.ptr(grad), _lib.ptr(g), _lib.ptr(s_flat.contiguous() if expand_state else s_flat),
             _lib.DTYPE_CODES[state.dtype], n, k, _lib.int_array(qubits), batch, op_count,
-            s_stride, _lib.stream_handle(state.device))
+            s_stride, _lib.ptr(work) if work is not None else None,
+            _lib.stream_handle(state.device))
     _lib.check(rc, "operator gradient")
     return grad
 

@@ -100,7 +100,17 @@ def test_validation_errors_of_layer_permutation_sampling_and_constructors(lib):
     assert lib.ua_sample(None, a, c64, 4, -1, 0, 0, b, None) == bad
     assert lib.ua_sample(None, a, c64, 41, 5, 0, 0, b, None) == bad
     assert lib.ua_sample_workspace_bytes(41) == -1
-    assert lib.ua_sample_workspace_bytes(12) == 8 * (4096 + 2)
+    assert lib.ua_sample_workspace_bytes(12) == 8 * (4096 + 2 + 256 + 16)
+    # operator gradient: 2048 slices of 2^23 pairs for a 1-qubit operator, 16
+    # slices x 256 blocks for a 6-qubit one, none when k == n
+    assert lib.ua_operator_gradient_workspace_bytes(24, 1, 1, 1) == 2048 * 4 * 16
+    assert lib.ua_operator_gradient_workspace_bytes(24, 6, 1, 1) == 16 * 4096 * 16
+    assert lib.ua_operator_gradient_workspace_bytes(10, 10, 1, 1) == 0
+    assert lib.ua_operator_gradient_workspace_bytes(4, 5, 1, 1) == -1
+    assert lib.ua_operator_gradient_workspace_bytes(4, 1, 3, 2) == -1
+    assert lib.ua_operator_gradient(a, b, a, c64, 24, 1, _lib.int_array([0]), 1, 1, 0, None,
+                                    None) == bad
+    assert "workspace" in lib.ua_last_error().decode()
     assert lib.ua_rand_state(a, c64, 1, 1, 0, b, None) == bad
     assert lib.ua_fill_state(a, c64, 4, 1, 1, 16, None) == bad
     assert lib.ua_fill_state(a, c64, 4, 1, 2, 0, None) == bad

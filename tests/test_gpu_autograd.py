@@ -44,6 +44,42 @@ def test_apply_operator_autograd(device, ob, sb, qubits):
         assert torch.allclose(r, g, atol=1e-10)
 
 
+# Larger states: the operator-gradient reduction is split across slices of
+# the groups (n=14..20), across operators (op batch 2 with 2^19 groups each),
+# and tiled 4 x 4 for k >= 3 (k = 3 with 16 slices, k = 5 and k = 6 blocked).
+@pytest.mark.parametrize("n,qubits,ob,sb", [(14, (5,), (), ()), (14, (13, 0), (), (3,)),
+                                            (18, (2, 9, 17), (), ()),
+                                            (20, (0,), (2,), (2,)),
+                                            (16, (0, 4, 8, 12, 15), (2,), (2,)),
+                                            (13, (1, 3, 5, 7, 9, 11), (), ()),
+                                            (12, (3,), (4,), ())])
+def test_operator_gradient_large_states(device, n, qubits, ob, sb):
+    torch.manual_seed(n + len(qubits))
+    k = len(qubits)
+    op = torch.randn(*ob, 2 ** k, 2 ** k, dtype=torch.complex128)
+    st = torch.randn(*sb, 2 ** n, dtype=torch.complex128)
+    ref = _grads_cpu(lambda o, s: T.apply_operator(o, qubits, s), op, st)
+    got = _grads_gpu(lambda o, s: sim.apply_operator(o, qubits, s), device, op, st)
+    for r, g in zip(ref, got):
+        scale = float(r.abs().max())
+        assert float((r - g).abs().max()) <= 1e-12 * scale
+
+
+def test_operator_gradient_complex64_large(device):
+    torch.manual_seed(7)
+    n = 20
+    op = torch.randn(2, 2, dtype=torch.complex128)
+    st = torch.randn(2 ** n, dtype=torch.complex128)
+    ref = _grads_cpu(lambda o, s: T.apply_operator(o, [6], s),
+                     op.to(torch.complex64).to(torch.complex128),
+                     st.to(torch.complex64).to(torch.complex128))
+    got = _grads_gpu(lambda o, s: sim.apply_operator(o, [6], s), device,
+                     op.to(torch.complex64), st.to(torch.complex64))
+    for r, g in zip(ref, got):
+        scale = float(r.abs().max())
+        assert float((r - g.to(torch.complex128)).abs().max()) <= 1e-5 * scale
+
+
 @pytest.mark.parametrize("ashape,sshape", [((16,), (3, 16)), ((3, 16), (3, 16)), ((3, 1), (3, 16))])
 def test_apply_phase_autograd(device, ashape, sshape):
     torch.manual_seed(2)
