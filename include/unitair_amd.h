@@ -164,10 +164,11 @@ int ua_apply_controlled_flip(void* out, const void* state, int dtype, int num_qu
  * abs_squared, then torch.distributions.Categorical(probs).sample): draws
  * num_samples outcomes x with probability |state[x]|^2 / sum |state|^2 into
  * out (int64).  Float64 prefix sums; uniforms from Philox4x32-10 with key
- * `seed` and counters offset .. offset + num_samples - 1.  workspace:
- * ua_sample_workspace_bytes(num_qubits) bytes, 8-byte aligned.  One state.
+ * `seed` and counters offset .. offset + num_samples - 1 (num_samples below
+ * 2^31).  workspace: ua_sample_workspace_bytes(num_qubits, num_samples)
+ * bytes, 8-byte aligned.  One state.
  */
-int64_t ua_sample_workspace_bytes(int num_qubits);
+int64_t ua_sample_workspace_bytes(int num_qubits, int64_t num_samples);
 int ua_sample(int64_t* out, const void* state, int dtype, int num_qubits, int64_t num_samples,
               uint64_t seed, uint64_t offset, void* workspace, ua_stream_t stream);
 

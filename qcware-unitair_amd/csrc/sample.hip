@@ -2,21 +2,27 @@
 // draw steps of unitair.simulation.measure (unitair/simulation/measurement.py:
 // 9-71: abs_squared, then torch.distributions.Categorical(probs).sample).
 //
-// 1. scan_chunks_kernel: |psi_x|^2 in float64, inclusive prefix sums inside
-//    chunks of 4096 amplitudes (cum), the prefix at the end of every row of
-//    16 (row_cum) and of every group of 16 rows (group_cum), and each
-//    chunk's total (chunk_sum).
+// Sample s takes u = U(s) * total, U(s) a 53-bit uniform in [0, 1) from
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11) with counter (s + offset, 0, 0, 0) and key (seed
+// lo, seed hi).  The outcome is the first index whose inclusive prefix of
+// |psi|^2 exceeds u, with prefixes formed in float64 inside chunks of 4096
+// amplitudes and then across chunks:
+// 1. scan_chunks_kernel: each chunk's total (chunk_sum).
 // 2. scan_totals_kernel: one workgroup turns chunk_sum into inclusive chunk
-//    prefixes (chunk_prefix).
-// 3. draw_kernel: sample s takes u = U(s) * total, U(s) a 53-bit uniform in
-//    [0, 1) from Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
-//    numbers: as easy as 1, 2, 3", SC'11) with counter (s + offset, 0, 0, 0)
-//    and key (seed lo, seed hi); the outcome is the first index whose
-//    inclusive prefix exceeds u: a binary search over chunk_prefix, then over
-//    that chunk's group_cum, row_cum and cum (branch-free, four samples per
-//    thread in lockstep).
-// Categorical(probs) normalises by the sum of probs; so does drawing u in
-// [0, total).
+//    prefixes.
+// 3. bucket_kernel: per sample, the first chunk whose prefix exceeds u (the
+//    chunk prefixes staged in LDS), and a slot in that chunk's bucket
+//    (vector atomic on the chunk's count).
+// 4. count_scan_kernel / scatter_kernel: bucket starts, then the sample ids
+//    in bucket order.
+// 5. draw_chunk_kernel: one workgroup per chunk that holds samples streams
+//    the chunk from the state, rebuilds its in-chunk prefixes in LDS with the
+//    same code as step 1 (so the values agree bit for bit), and searches
+//    there for each of its samples.
+// The state is read twice in order; no prefix array goes through HBM and no
+// sample makes a scattered read.  Categorical(probs) normalises by the sum
+// of probs; so does drawing u in [0, total).
 #include "ua_common.h"
 
 namespace ua {
@@ -25,9 +31,7 @@ namespace {
 constexpr int kChunkBits = 12;
 constexpr int kChunk = 1 << kChunkBits;       // amplitudes per chunk
 constexpr int kScanThreads = 256;
-constexpr int kPerThread = kChunk / kScanThreads;  // 16 amplitudes per row
-constexpr int kFan = 16;                           // rows per group
-constexpr int kGroups = kScanThreads / kFan;       // groups per chunk
+constexpr int kPerThread = kChunk / kScanThreads;  // 16 amplitudes per thread
 
 template <typename T> __device__ __forceinline__ double prob(T v);
 template <> __device__ __forceinline__ double prob(float v) { return double(v) * double(v); }
@@ -37,22 +41,21 @@ template <> __device__ __forceinline__ double prob(float2 v) {
 }
 template <> __device__ __forceinline__ double prob(double2 v) { return v.x * v.x + v.y * v.y; }
 
-// The chunk passes through LDS so that global loads and stores stay
-// coalesced while each thread scans 16 contiguous entries; rows of 16 are
-// padded to 17 doubles to spread the threads' LDS accesses over the banks.
+// The chunk passes through LDS so that global loads stay coalesced while
+// each thread scans 16 contiguous entries; rows of 16 are padded to 17
+// doubles to spread the threads' LDS accesses over the banks.
 constexpr int kRow = kPerThread + 1;
 
 __device__ __forceinline__ int lds_slot(int i) { return (i / kPerThread) * kRow + i % kPerThread; }
 
+// Chunk `c` of the state: leaves the inclusive in-chunk prefix of entry i
+// at vals[lds_slot(i)] (entries at or beyond N count as 0) and returns the
+// calling thread's inclusive total; thread kScanThreads-1 holds the chunk's.
 template <typename T>
-__global__ __launch_bounds__(kScanThreads) void scan_chunks_kernel(const T* state, int64_t N,
-                                                                   double* cum, double* row_cum,
-                                                                   double* group_cum,
-                                                                   double* chunk_sum) {
-  __shared__ double vals[kScanThreads * kRow];
-  __shared__ double wave_tot[kScanThreads / 64];
+__device__ double chunk_prefix_lds(const T* state, int64_t N, int64_t c, double* vals,
+                                   double* wave_tot) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t c0 = int64_t(blockIdx.x) * kChunk;
+  const int64_t c0 = c * kChunk;
 #pragma unroll
   for (int j = 0; j < kPerThread; ++j) {
     const int i = j * kScanThreads + tid;
@@ -79,18 +82,17 @@ __global__ __launch_bounds__(kScanThreads) void scan_chunks_kernel(const T* stat
   for (int w = 0; w < wave; ++w) before += wave_tot[w];
 #pragma unroll
   for (int j = 0; j < kPerThread; ++j) vals[tid * kRow + j] = before + local[j];
-  // the row's last in-chunk prefix, bit for bit the cum entry it ends on
-  row_cum[int64_t(blockIdx.x) * kScanThreads + tid] = before + local[kPerThread - 1];
-  // and every 16th row's, for the level above
-  if (tid % kFan == kFan - 1)
-    group_cum[int64_t(blockIdx.x) * kGroups + tid / kFan] = before + local[kPerThread - 1];
-  if (tid == kScanThreads - 1) chunk_sum[blockIdx.x] = before + run;
   __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kPerThread; ++j) {
-    const int i = j * kScanThreads + tid;
-    if (c0 + i < N) cum[c0 + i] = vals[lds_slot(i)];
-  }
+  return before + run;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void scan_chunks_kernel(const T* state, int64_t N,
+                                                                   double* chunk_sum) {
+  __shared__ double vals[kScanThreads * kRow];
+  __shared__ double wave_tot[kScanThreads / 64];
+  const double t = chunk_prefix_lds(state, N, blockIdx.x, vals, wave_tot);
+  if (threadIdx.x == kScanThreads - 1) chunk_sum[blockIdx.x] = t;
 }
 
 // One workgroup: inclusive prefix of the chunk totals, 1024 at a time.
@@ -142,35 +144,43 @@ __device__ __forceinline__ void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uin
   }
 }
 
-// LDS_PREFIX: the chunk prefixes (at most kLdsChunks) are staged in LDS once
-// per workgroup, so the first half of every search stays on chip.
-constexpr int kLdsChunks = 4096;  // 32 KiB of doubles (states up to 24 qubits)
-// Samples per thread, searched in lockstep: their dependent loads overlap.
-constexpr int kLanesPerThread = 4;
+// u = U(s) * total, kept below total: the product can round up to it, and
+// the first prefix above a value just below total is the last outcome with
+// nonzero probability (an outcome of probability 0 adds nothing to the
+// prefix, so it is never the first to exceed anything).
+__device__ __forceinline__ double draw_u(int64_t s, uint64_t seed, uint64_t offset,
+                                         double total) {
+  const uint64_t ctr = uint64_t(s) + offset;
+  uint32_t c[4] = {uint32_t(ctr), uint32_t(ctr >> 32), 0u, 0u};
+  philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
+  const uint64_t bits53 = (uint64_t(c[0]) << 21) ^ (uint64_t(c[1]) >> 11);
+  const double u = double(bits53 & ((uint64_t(1) << 53) - 1)) * 0x1.0p-53 * total;
+  return fmin(u, nextafter(total, 0.0));
+}
 
 // First index i in [0, m) with a[i] > u, or m - 1 if there is none; m is a
 // power of two.  Branch-free: log2(m) steps for every u.
-// The S loads of a step are all issued before any is compared, so that the
-// S searches' memory latencies overlap.
-template <int S>
-__device__ __forceinline__ void search_pow2(const double* __restrict__ a, int64_t m,
-                                            const double (&u)[S], int64_t (&at)[S]) {
-  for (int64_t half = m >> 1; half >= 1; half >>= 1) {
-    double v[S];
-#pragma unroll
-    for (int i = 0; i < S; ++i) v[i] = a[at[i] + half - 1];
-#pragma unroll
-    for (int i = 0; i < S; ++i) at[i] += (v[i] <= u[i]) ? half : 0;
-  }
+__device__ __forceinline__ int64_t first_above(const double* a, int64_t m, double u) {
+  int64_t at = 0;
+  for (int64_t half = m >> 1; half >= 1; half >>= 1) at += (a[at + half - 1] <= u) ? half : 0;
+  return at;
 }
 
+__global__ __launch_bounds__(kBlock) void zero_counts_kernel(uint32_t* count, int64_t chunks) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < chunks) count[i] = 0u;
+}
+
+// LDS_PREFIX: the chunk prefixes (at most kLdsChunks) are staged in LDS once
+// per workgroup.
+constexpr int kLdsChunks = 4096;  // 32 KiB of doubles (states up to 24 qubits)
+
 template <bool LDS_PREFIX>
-__global__ __launch_bounds__(kBlock) void draw_kernel(int64_t* out, int64_t num_samples,
-                                                     const double* cum, const double* row_cum,
-                                                     const double* group_cum,
-                                                     const double* gprefix, int64_t chunks,
-                                                     int64_t chunk_len, uint64_t seed,
-                                                     uint64_t offset) {
+__global__ __launch_bounds__(kBlock) void bucket_kernel(int64_t num_samples,
+                                                       const double* gprefix, int64_t chunks,
+                                                       uint64_t seed, uint64_t offset,
+                                                       uint32_t* cid, uint32_t* slot,
+                                                       uint32_t* count) {
   extern __shared__ double sprefix[];
   const double* prefix = gprefix;
   if constexpr (LDS_PREFIX) {
@@ -191,62 +201,99 @@ __global__ __launch_bounds__(kBlock) void draw_kernel(int64_t* out, int64_t num_
     __syncthreads();
     prefix = sprefix;
   }
-  constexpr int S = kLanesPerThread;
-  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   const double total = prefix[chunks - 1];
-  for (int64_t s0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s0 < num_samples;
-       s0 += S * stride) {
-    double u[S];
-    int64_t lo[S];
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      const uint64_t ctr = uint64_t(s0 + i * stride) + offset;
-      uint32_t c[4] = {uint32_t(ctr), uint32_t(ctr >> 32), 0u, 0u};
-      philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
-      const uint64_t bits53 = (uint64_t(c[0]) << 21) ^ (uint64_t(c[1]) >> 11);
-      u[i] = double(bits53 & ((uint64_t(1) << 53) - 1)) * 0x1.0p-53 * total;
-      lo[i] = 0;
-    }
-    // first chunk whose inclusive prefix exceeds u, then the first index of
-    // that chunk whose inclusive (in-chunk) prefix exceeds what is left of
-    // u, down a tree of fan-out 16: the group of 16 rows by the groups'
-    // last prefixes, the row by the rows' last prefixes, the entry inside
-    // the row -- one 128-byte line per level.  Rounding can leave u at a
-    // chunk's total: every level ends on its last entry then.
-    search_pow2<S>(prefix, chunks, u, lo);
-    const int64_t row_len = chunk_len < kPerThread ? chunk_len : kPerThread;
-    const int64_t rows = chunk_len / row_len;
-    const int64_t groups = rows >= kFan ? rows / kFan : 1;
-    const int64_t rows_per_group = rows / groups;
-    double ul[S];
-    int64_t at[S];
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      ul[i] = u[i] - (lo[i] > 0 ? prefix[lo[i] - 1] : 0.0);
-      at[i] = lo[i] * kGroups;
-    }
-    search_pow2<S>(group_cum, groups, ul, at);
-#pragma unroll
-    for (int i = 0; i < S; ++i)
-      at[i] = lo[i] * kScanThreads + (at[i] - lo[i] * kGroups) * rows_per_group;
-    search_pow2<S>(row_cum, rows_per_group, ul, at);
-#pragma unroll
-    for (int i = 0; i < S; ++i) at[i] = lo[i] * chunk_len + (at[i] - lo[i] * kScanThreads) * row_len;
-    search_pow2<S>(cum, row_len, ul, at);
-#pragma unroll
-    for (int i = 0; i < S; ++i)
-      if (s0 + i * stride < num_samples) out[s0 + i * stride] = at[i];
+  const int64_t stride = int64_t(gridDim.x) * kBlock;
+  for (int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x; s < num_samples; s += stride) {
+    const int64_t c = first_above(prefix, chunks, draw_u(s, seed, offset, total));
+    cid[s] = uint32_t(c);
+    slot[s] = atomicAdd(&count[c], 1u);
   }
+}
+
+// One workgroup: exclusive prefix of the bucket counts, 1024 at a time.
+__global__ __launch_bounds__(1024) void count_scan_kernel(const uint32_t* count, int64_t chunks,
+                                                         uint32_t* start) {
+  __shared__ uint32_t wave_tot[16];
+  __shared__ uint32_t carry_s;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) carry_s = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < chunks; base += 1024) {
+    const int64_t i = base + tid;
+    const uint32_t v = i < chunks ? count[i] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) wave_tot[wave] = x;
+    __syncthreads();
+    uint32_t before = carry_s;
+    for (int w = 0; w < wave; ++w) before += wave_tot[w];
+    if (i < chunks) start[i] = before + x - v;
+    __syncthreads();
+    if (tid == 1023) carry_s = before + x;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void scatter_kernel(int64_t num_samples, const uint32_t* cid,
+                                                        const uint32_t* slot,
+                                                        const uint32_t* start, uint32_t* bucket) {
+  const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (s < num_samples) bucket[start[cid[s]] + slot[s]] = uint32_t(s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void draw_chunk_kernel(
+    int64_t* out, const T* state, int64_t N, int64_t chunk_len, const double* prefix,
+    int64_t chunks, const uint32_t* count, const uint32_t* start, const uint32_t* bucket,
+    uint64_t seed, uint64_t offset) {
+  __shared__ double vals[kScanThreads * kRow];
+  __shared__ double wave_tot[kScanThreads / 64];
+  const int64_t c = blockIdx.x;
+  const uint32_t n_c = count[c];
+  if (n_c == 0) return;  // the whole workgroup leaves together
+  chunk_prefix_lds(state, N, c, vals, wave_tot);
+  const double total = prefix[chunks - 1];
+  const double below = c > 0 ? prefix[c - 1] : 0.0;
+  const uint32_t first = start[c];
+  const double below_chunk_total = nextafter(vals[lds_slot(int(chunk_len - 1))], 0.0);
+  for (uint32_t j = threadIdx.x; j < n_c; j += kScanThreads) {
+    const int64_t s = bucket[first + j];
+    // what is left of u inside the chunk, kept below the chunk's total (the
+    // subtraction can round up to it) for the same reason as u itself
+    const double ul = fmin(draw_u(s, seed, offset, total) - below, below_chunk_total);
+    int64_t at = 0;
+    for (int64_t half = chunk_len >> 1; half >= 1; half >>= 1)
+      at += (vals[lds_slot(int(at + half - 1))] <= ul) ? half : 0;
+    out[s] = c * chunk_len + at;
+  }
+}
+
+struct SampleLayout {
+  int64_t chunks;
+  int64_t bytes;
+};
+
+SampleLayout sample_layout(int num_qubits, int64_t num_samples) {
+  const int64_t N = int64_t(1) << num_qubits;
+  SampleLayout l;
+  l.chunks = (N + kChunk - 1) / kChunk;
+  // chunk_sum, prefix (float64); count, start (uint32); cid, slot, bucket (uint32)
+  const int64_t b = 8 * 2 * l.chunks + 4 * 2 * l.chunks + 4 * 3 * num_samples;
+  l.bytes = (b + 7) / 8 * 8;
+  return l;
 }
 
 }  // namespace
 }  // namespace ua
 
-extern "C" int64_t ua_sample_workspace_bytes(int num_qubits) {
-  if (num_qubits < 0 || num_qubits > 40) return -1;
-  const int64_t N = int64_t(1) << num_qubits;
-  const int64_t chunks = (N + ua::kChunk - 1) / ua::kChunk;
-  return 8 * (N + chunks * (2 + ua::kScanThreads + ua::kGroups));
+extern "C" int64_t ua_sample_workspace_bytes(int num_qubits, int64_t num_samples) {
+  if (num_qubits < 0 || num_qubits > 40 || num_samples < 0 || num_samples > 0x7fffffffLL)
+    return -1;
+  return ua::sample_layout(num_qubits, num_samples).bytes;
 }
 
 extern "C" int ua_sample(int64_t* out, const void* state, int dtype, int num_qubits,
@@ -255,51 +302,61 @@ extern "C" int ua_sample(int64_t* out, const void* state, int dtype, int num_qub
   using namespace ua;
   UA_CHECK_ARG(num_qubits >= 0 && num_qubits <= 40, "num_qubits=%d out of range", num_qubits);
   UA_CHECK_ARG(dtype >= UA_FLOAT32 && dtype <= UA_COMPLEX128, "bad dtype %d", dtype);
-  UA_CHECK_ARG(num_samples >= 0, "negative num_samples");
+  UA_CHECK_ARG(num_samples >= 0 && num_samples <= 0x7fffffffLL,
+               "num_samples=%lld out of range (0 .. 2^31 - 1)", (long long)num_samples);
   if (num_samples == 0) return UA_OK;
   UA_CHECK_ARG(out && state && workspace, "NULL pointer");
   UA_CHECK_ARG(reinterpret_cast<uintptr_t>(workspace) % 8 == 0, "workspace must be 8-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int64_t N = int64_t(1) << num_qubits;
-  const int64_t chunks = (N + kChunk - 1) / kChunk;
-  double* cum = static_cast<double*>(workspace);
-  double* chunk_sum = cum + N;
+  const int64_t chunks = sample_layout(num_qubits, num_samples).chunks;
+  const int64_t chunk_len = N < kChunk ? N : kChunk;  // chunks and chunk_len are powers of two
+  double* chunk_sum = static_cast<double*>(workspace);
   double* prefix = chunk_sum + chunks;
-  double* row_cum = prefix + chunks;              // kScanThreads rows per chunk
-  double* group_cum = row_cum + chunks * kScanThreads;  // kGroups groups per chunk
+  uint32_t* count = reinterpret_cast<uint32_t*>(prefix + chunks);
+  uint32_t* start = count + chunks;
+  uint32_t* cid = start + chunks;
+  uint32_t* slot = cid + num_samples;
+  uint32_t* bucket = slot + num_samples;
+
+  zero_counts_kernel<<<grid_for(chunks), kBlock, 0, s>>>(count, chunks);
+  if (int rc = check_launch("zero_counts_kernel")) return rc;
+#define UA_SCAN(T) \
+  scan_chunks_kernel<T><<<unsigned(chunks), kScanThreads, 0, s>>>(static_cast<const T*>(state), N, chunk_sum)
   switch (dtype) {
-    case UA_FLOAT32:
-      scan_chunks_kernel<float><<<unsigned(chunks), kScanThreads, 0, s>>>(
-          static_cast<const float*>(state), N, cum, row_cum, group_cum, chunk_sum);
-      break;
-    case UA_FLOAT64:
-      scan_chunks_kernel<double><<<unsigned(chunks), kScanThreads, 0, s>>>(
-          static_cast<const double*>(state), N, cum, row_cum, group_cum, chunk_sum);
-      break;
-    case UA_COMPLEX64:
-      scan_chunks_kernel<float2><<<unsigned(chunks), kScanThreads, 0, s>>>(
-          static_cast<const float2*>(state), N, cum, row_cum, group_cum, chunk_sum);
-      break;
-    default:
-      scan_chunks_kernel<double2><<<unsigned(chunks), kScanThreads, 0, s>>>(
-          static_cast<const double2*>(state), N, cum, row_cum, group_cum, chunk_sum);
-      break;
+    case UA_FLOAT32: UA_SCAN(float); break;
+    case UA_FLOAT64: UA_SCAN(double); break;
+    case UA_COMPLEX64: UA_SCAN(float2); break;
+    default: UA_SCAN(double2); break;
   }
+#undef UA_SCAN
   if (int rc = check_launch("scan_chunks_kernel")) return rc;
   scan_totals_kernel<<<1, 1024, 0, s>>>(chunk_sum, chunks, prefix);
   if (int rc = check_launch("scan_totals_kernel")) return rc;
-  // chunks and chunk_len are powers of two (N is)
-  const int64_t chunk_len = N < kChunk ? N : kChunk;
-  const int64_t per_block = int64_t(kBlock) * kLanesPerThread;
-  const int64_t blocks = (num_samples + per_block - 1) / per_block;
+  const int64_t blocks = (num_samples + kBlock - 1) / kBlock;
   if (chunks <= kLdsChunks) {
-    // enough workgroups to fill the GPU, each drawing many samples
-    const unsigned grid = unsigned(blocks < 1024 ? blocks : 1024);
-    draw_kernel<true><<<grid, kBlock, size_t(chunks) * sizeof(double), s>>>(
-        out, num_samples, cum, row_cum, group_cum, prefix, chunks, chunk_len, seed, offset);
+    const unsigned grid = unsigned(blocks < 2048 ? blocks : 2048);
+    bucket_kernel<true><<<grid, kBlock, size_t(chunks) * sizeof(double), s>>>(
+        num_samples, prefix, chunks, seed, offset, cid, slot, count);
   } else {
-    draw_kernel<false><<<unsigned(blocks < 8192 ? blocks : 8192), kBlock, 0, s>>>(
-        out, num_samples, cum, row_cum, group_cum, prefix, chunks, chunk_len, seed, offset);
+    bucket_kernel<false><<<unsigned(blocks < 8192 ? blocks : 8192), kBlock, 0, s>>>(
+        num_samples, prefix, chunks, seed, offset, cid, slot, count);
   }
-  return check_launch("draw_kernel");
+  if (int rc = check_launch("bucket_kernel")) return rc;
+  count_scan_kernel<<<1, 1024, 0, s>>>(count, chunks, start);
+  if (int rc = check_launch("count_scan_kernel")) return rc;
+  scatter_kernel<<<unsigned(blocks), kBlock, 0, s>>>(num_samples, cid, slot, start, bucket);
+  if (int rc = check_launch("scatter_kernel")) return rc;
+#define UA_DRAW(T)                                                                           \
+  draw_chunk_kernel<T><<<unsigned(chunks), kScanThreads, 0, s>>>(                            \
+      out, static_cast<const T*>(state), N, chunk_len, prefix, chunks, count, start, bucket, \
+      seed, offset)
+  switch (dtype) {
+    case UA_FLOAT32: UA_DRAW(float); break;
+    case UA_FLOAT64: UA_DRAW(double); break;
+    case UA_COMPLEX64: UA_DRAW(float2); break;
+    default: UA_DRAW(double2); break;
+  }
+#undef UA_DRAW
+  return check_launch("draw_chunk_kernel");
 }

@@ -39,7 +39,10 @@ def sample_indices(state: torch.Tensor, num_samples: int,
     if num_samples == 0:
         return out
     lib = _lib.load()
-    ws = torch.empty(lib.ua_sample_workspace_bytes(n), dtype=torch.uint8, device=state.device)
+    nbytes = lib.ua_sample_workspace_bytes(n, int(num_samples))
+    if nbytes < 0:
+        raise ValueError(f"measure: num_samples={num_samples} out of range (0 .. 2^31 - 1)")
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=state.device)
     seed = int(torch.randint(0, 2 ** 62, (1,), generator=generator).item())
     with on_device(state.device):
         rc = lib.ua_sample(_lib.ptr(out), _lib.ptr(state), _lib.DTYPE_CODES[state.dtype], n,
