@@ -13,9 +13,11 @@
 //    prefixes.
 // 3. bucket_kernel: per sample, the first chunk whose prefix exceeds u (the
 //    chunk prefixes staged in LDS), and a slot in that chunk's bucket
-//    (vector atomic on the chunk's count).
-// 4. count_scan_kernel / scatter_kernel: bucket starts, then the sample ids
-//    in bucket order.
+//    (vector atomic on one of the chunk's kSets sub-counts, picked by the
+//    workgroup, so that fewer atomics meet on one address).
+// 4. count_scan_kernel / key_start_kernel / scatter_kernel: bucket starts
+//    per chunk, then per sub-bucket (a chunk's sub-buckets one after
+//    another), then the sample ids in bucket order.
 // 5. draw_chunk_kernel: one workgroup per chunk that holds samples streams
 //    the chunk from the state, rebuilds its in-chunk prefixes in LDS with the
 //    same code as step 1 (so the values agree bit for bit), and searches
@@ -32,6 +34,7 @@ constexpr int kChunkBits = 12;
 constexpr int kChunk = 1 << kChunkBits;       // amplitudes per chunk
 constexpr int kScanThreads = 256;
 constexpr int kPerThread = kChunk / kScanThreads;  // 16 amplitudes per thread
+constexpr int kSets = 8;                           // sub-counts per chunk
 
 template <typename T> __device__ __forceinline__ double prob(T v);
 template <> __device__ __forceinline__ double prob(float v) { return double(v) * double(v); }
@@ -166,9 +169,9 @@ __device__ __forceinline__ int64_t first_above(const double* a, int64_t m, doubl
   return at;
 }
 
-__global__ __launch_bounds__(kBlock) void zero_counts_kernel(uint32_t* count, int64_t chunks) {
+__global__ __launch_bounds__(kBlock) void zero_counts_kernel(uint32_t* count, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i < chunks) count[i] = 0u;
+  if (i < n) count[i] = 0u;
 }
 
 // LDS_PREFIX: the chunk prefixes (at most kLdsChunks) are staged in LDS once
@@ -205,12 +208,14 @@ __global__ __launch_bounds__(kBlock) void bucket_kernel(int64_t num_samples,
   const int64_t stride = int64_t(gridDim.x) * kBlock;
   for (int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x; s < num_samples; s += stride) {
     const int64_t c = first_above(prefix, chunks, draw_u(s, seed, offset, total));
-    cid[s] = uint32_t(c);
-    slot[s] = atomicAdd(&count[c], 1u);
+    const uint32_t key = uint32_t(c * kSets + (blockIdx.x & (kSets - 1)));
+    cid[s] = key;
+    slot[s] = atomicAdd(&count[key], 1u);
   }
 }
 
-// One workgroup: exclusive prefix of the bucket counts, 1024 at a time.
+// One workgroup: exclusive prefix over chunks of the bucket sizes (each the
+// sum of the chunk's kSets sub-counts), 1024 chunks at a time.
 __global__ __launch_bounds__(1024) void count_scan_kernel(const uint32_t* count, int64_t chunks,
                                                          uint32_t* start) {
   __shared__ uint32_t wave_tot[16];
@@ -220,7 +225,11 @@ __global__ __launch_bounds__(1024) void count_scan_kernel(const uint32_t* count,
   __syncthreads();
   for (int64_t base = 0; base < chunks; base += 1024) {
     const int64_t i = base + tid;
-    const uint32_t v = i < chunks ? count[i] : 0u;
+    uint32_t v = 0;
+    if (i < chunks) {
+#pragma unroll
+      for (int k = 0; k < kSets; ++k) v += count[i * kSets + k];
+    }
     uint32_t x = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -238,27 +247,43 @@ __global__ __launch_bounds__(1024) void count_scan_kernel(const uint32_t* count,
   }
 }
 
+// A chunk's bucket holds its sub-buckets in order.  One thread per chunk
+// turns the chunk's kSets sub-counts into the sub-buckets' starts, in place.
+__global__ __launch_bounds__(kBlock) void key_start_kernel(uint32_t* count, const uint32_t* start,
+                                                          int64_t chunks) {
+  const int64_t c = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (c >= chunks) return;
+  uint32_t at = start[c];
+#pragma unroll
+  for (int k = 0; k < kSets; ++k) {
+    const uint32_t n = count[c * kSets + k];
+    count[c * kSets + k] = at;
+    at += n;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void scatter_kernel(int64_t num_samples, const uint32_t* cid,
                                                         const uint32_t* slot,
-                                                        const uint32_t* start, uint32_t* bucket) {
+                                                        const uint32_t* key_start,
+                                                        uint32_t* bucket) {
   const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (s < num_samples) bucket[start[cid[s]] + slot[s]] = uint32_t(s);
+  if (s < num_samples) bucket[key_start[cid[s]] + slot[s]] = uint32_t(s);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void draw_chunk_kernel(
     int64_t* out, const T* state, int64_t N, int64_t chunk_len, const double* prefix,
-    int64_t chunks, const uint32_t* count, const uint32_t* start, const uint32_t* bucket,
+    int64_t chunks, int64_t num_samples, const uint32_t* start, const uint32_t* bucket,
     uint64_t seed, uint64_t offset) {
   __shared__ double vals[kScanThreads * kRow];
   __shared__ double wave_tot[kScanThreads / 64];
   const int64_t c = blockIdx.x;
-  const uint32_t n_c = count[c];
+  const uint32_t first = start[c];
+  const uint32_t n_c = uint32_t(c + 1 < chunks ? start[c + 1] : num_samples) - first;
   if (n_c == 0) return;  // the whole workgroup leaves together
   chunk_prefix_lds(state, N, c, vals, wave_tot);
   const double total = prefix[chunks - 1];
   const double below = c > 0 ? prefix[c - 1] : 0.0;
-  const uint32_t first = start[c];
   const double below_chunk_total = nextafter(vals[lds_slot(int(chunk_len - 1))], 0.0);
   for (uint32_t j = threadIdx.x; j < n_c; j += kScanThreads) {
     const int64_t s = bucket[first + j];
@@ -281,8 +306,9 @@ SampleLayout sample_layout(int num_qubits, int64_t num_samples) {
   const int64_t N = int64_t(1) << num_qubits;
   SampleLayout l;
   l.chunks = (N + kChunk - 1) / kChunk;
-  // chunk_sum, prefix (float64); count, start (uint32); cid, slot, bucket (uint32)
-  const int64_t b = 8 * 2 * l.chunks + 4 * 2 * l.chunks + 4 * 3 * num_samples;
+  // chunk_sum, prefix (float64); count (uint32, kSets per chunk); start
+  // (uint32 per chunk); cid, slot, bucket (uint32 per sample)
+  const int64_t b = 8 * 2 * l.chunks + 4 * (kSets + 1) * l.chunks + 4 * 3 * num_samples;
   l.bytes = (b + 7) / 8 * 8;
   return l;
 }
@@ -313,13 +339,14 @@ extern "C" int ua_sample(int64_t* out, const void* state, int dtype, int num_qub
   const int64_t chunk_len = N < kChunk ? N : kChunk;  // chunks and chunk_len are powers of two
   double* chunk_sum = static_cast<double*>(workspace);
   double* prefix = chunk_sum + chunks;
+  const int64_t keys = chunks * kSets;  // (chunk, sub-count) pairs, chunk-major
   uint32_t* count = reinterpret_cast<uint32_t*>(prefix + chunks);
-  uint32_t* start = count + chunks;
+  uint32_t* start = count + keys;      // one per chunk
   uint32_t* cid = start + chunks;
   uint32_t* slot = cid + num_samples;
   uint32_t* bucket = slot + num_samples;
 
-  zero_counts_kernel<<<grid_for(chunks), kBlock, 0, s>>>(count, chunks);
+  zero_counts_kernel<<<grid_for(keys), kBlock, 0, s>>>(count, keys);
   if (int rc = check_launch("zero_counts_kernel")) return rc;
 #define UA_SCAN(T) \
   scan_chunks_kernel<T><<<unsigned(chunks), kScanThreads, 0, s>>>(static_cast<const T*>(state), N, chunk_sum)
@@ -345,11 +372,14 @@ extern "C" int ua_sample(int64_t* out, const void* state, int dtype, int num_qub
   if (int rc = check_launch("bucket_kernel")) return rc;
   count_scan_kernel<<<1, 1024, 0, s>>>(count, chunks, start);
   if (int rc = check_launch("count_scan_kernel")) return rc;
-  scatter_kernel<<<unsigned(blocks), kBlock, 0, s>>>(num_samples, cid, slot, start, bucket);
+  key_start_kernel<<<grid_for(chunks), kBlock, 0, s>>>(count, start, chunks);
+  if (int rc = check_launch("key_start_kernel")) return rc;
+  scatter_kernel<<<unsigned(blocks), kBlock, 0, s>>>(num_samples, cid, slot, count, bucket);
   if (int rc = check_launch("scatter_kernel")) return rc;
 #define UA_DRAW(T)                                                                           \
   draw_chunk_kernel<T><<<unsigned(chunks), kScanThreads, 0, s>>>(                            \
-      out, static_cast<const T*>(state), N, chunk_len, prefix, chunks, count, start, bucket, \
+      out, static_cast<const T*>(state), N, chunk_len, prefix, chunks, num_samples, start,   \
+      bucket,                                                                                \
       seed, offset)
   switch (dtype) {
     case UA_FLOAT32: UA_DRAW(float); break;

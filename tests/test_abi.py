@@ -102,10 +102,11 @@ def test_validation_errors_of_layer_permutation_sampling_and_constructors(lib):
     assert lib.ua_sample(None, a, c64, 4, 1 << 31, 0, 0, b, None) == bad
     assert lib.ua_sample_workspace_bytes(41, 5) == -1
     assert lib.ua_sample_workspace_bytes(12, 1 << 31) == -1
-    # chunk totals and prefixes (float64), bucket counts and starts (uint32)
-    # per 4096-amplitude chunk; chunk id, slot and bucket entry per sample
-    assert lib.ua_sample_workspace_bytes(12, 0) == 24
-    assert lib.ua_sample_workspace_bytes(20, 101) == 24 * 256 + 12 * 101 + 4
+    # chunk total and prefix (float64), 8 bucket sub-counts and a bucket
+    # start (uint32) per 4096-amplitude chunk; chunk key, slot and bucket
+    # entry per sample
+    assert lib.ua_sample_workspace_bytes(12, 0) == 56   # 52, rounded up to 8
+    assert lib.ua_sample_workspace_bytes(20, 101) == 52 * 256 + 12 * 101 + 4
     # operator gradient: 2048 slices of 2^23 pairs for a 1-qubit operator, 16
     # slices x 256 blocks for a 6-qubit one, none when k == n
     assert lib.ua_operator_gradient_workspace_bytes(24, 1, 1, 1) == 2048 * 4 * 16
