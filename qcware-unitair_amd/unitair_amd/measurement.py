@@ -34,19 +34,21 @@ def sample_indices(state: torch.Tensor, num_samples: int,
         # the reference's histogram step hashes sample.tolist() rows and
         # fails on batched states
         raise TypeError("measure: one state in vector layout, size (2^n,), is expected")
+    num_samples = int(num_samples)
+    if not 0 <= num_samples < 2 ** 31:
+        # sample ids and bucket slots are 32-bit on the device
+        raise ValueError(f"measure: num_samples={num_samples} out of range (0 .. 2^31 - 1)")
     state = prepare(state).contiguous()
-    out = torch.empty(int(num_samples), dtype=torch.int64, device=state.device)
+    out = torch.empty(num_samples, dtype=torch.int64, device=state.device)
     if num_samples == 0:
         return out
     lib = _lib.load()
-    nbytes = lib.ua_sample_workspace_bytes(n, int(num_samples))
-    if nbytes < 0:
-        raise ValueError(f"measure: num_samples={num_samples} out of range (0 .. 2^31 - 1)")
-    ws = torch.empty(nbytes, dtype=torch.uint8, device=state.device)
+    ws = torch.empty(lib.ua_sample_workspace_bytes(n, num_samples), dtype=torch.uint8,
+                     device=state.device)
     seed = int(torch.randint(0, 2 ** 62, (1,), generator=generator).item())
     with on_device(state.device):
         rc = lib.ua_sample(_lib.ptr(out), _lib.ptr(state), _lib.DTYPE_CODES[state.dtype], n,
-                           int(num_samples), seed, 0, _lib.ptr(ws),
+                           num_samples, seed, 0, _lib.ptr(ws),
                            _lib.stream_handle(state.device))
     _lib.check(rc, "measure")
     return out

@@ -105,6 +105,14 @@ def test_large_state_support(device):
     assert abs(frac - p_big) < 5 * np.sqrt(p_big * (1 - p_big) / k)
 
 
+def test_sample_count_limit_is_checked_before_any_allocation(device):
+    state = torch.ones(8, dtype=torch.complex64, device=device)
+    for bad in (2 ** 31, -1):
+        with pytest.raises(ValueError):
+            sample_indices(state, bad)
+    assert sample_indices(state, 0).shape == (0,)
+
+
 def test_measure_rejects_batches_and_cpu(device):
     with pytest.raises(TypeError):
         sim.measure(torch.ones(2, 8, dtype=torch.complex64, device=device), 10)
