@@ -402,7 +402,7 @@ __device__ __forceinline__ uint64_t gate_offset(int idx, const OpGradParams& p) 
   return o;
 }
 
-template <typename T, int RB, int CB>
+template <typename T, int RB, int CB, int UNR>
 __global__ __launch_bounds__(kBlock) void op_grad_partial_kernel(
     T* __restrict__ grad_op, double* __restrict__ part_out, const T* __restrict__ gout,
     const T* __restrict__ in, OpGradParams p) {
@@ -429,25 +429,36 @@ __global__ __launch_bounds__(kBlock) void op_grad_partial_kernel(
   for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int j = 0; j < CB; ++j) acc[i][j] = make_double2(0.0, 0.0);
-  for (int64_t u = u_begin + threadIdx.x; u < u_end; u += kBlock) {
-    const int64_t b = (p.op_count == 1) ? (u >> gbits) : o;
-    const uint64_t gi = uint64_t(u) & ((uint64_t(1) << gbits) - 1);
-    const uint64_t base = insert_zero_bits(gi, p.sorted, p.k);
-    const T* gb = gout + b * N + base;
-    const T* xb = in + b * p.in_bstride + base;
-    double2 g[RB], x[CB];
+  // UNR groups per iteration, all their loads issued before any product
+  for (int64_t u0 = u_begin + threadIdx.x; u0 < u_end; u0 += UNR * kBlock) {
+    T gr[UNR][RB], xr[UNR][CB];
 #pragma unroll
-    for (int i = 0; i < RB; ++i) g[i] = as_d2(gb[offr[i]]);
+    for (int q = 0; q < UNR; ++q) {
+      const int64_t u = u0 + q * kBlock;
+      if (u >= u_end) continue;
+      const int64_t b = (p.op_count == 1) ? (u >> gbits) : o;
+      const uint64_t gi = uint64_t(u) & ((uint64_t(1) << gbits) - 1);
+      const uint64_t base = insert_zero_bits(gi, p.sorted, p.k);
+      const T* gb = gout + b * N + base;
+      const T* xb = in + b * p.in_bstride + base;
 #pragma unroll
-    for (int j = 0; j < CB; ++j) x[j] = as_d2(xb[offc[j]]);
+      for (int i = 0; i < RB; ++i) gr[q][i] = gb[offr[i]];
 #pragma unroll
-    for (int i = 0; i < RB; ++i)
+      for (int j = 0; j < CB; ++j) xr[q][j] = xb[offc[j]];
+    }
 #pragma unroll
-      for (int j = 0; j < CB; ++j) {
-        // g * conj(x)
-        acc[i][j].x = fma(g[i].x, x[j].x, fma(g[i].y, x[j].y, acc[i][j].x));
-        acc[i][j].y = fma(g[i].y, x[j].x, fma(-g[i].x, x[j].y, acc[i][j].y));
-      }
+    for (int q = 0; q < UNR; ++q) {
+      if (u0 + q * kBlock >= u_end) continue;
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+          // g * conj(x)
+          const double2 g = as_d2(gr[q][i]), x = as_d2(xr[q][j]);
+          acc[i][j].x = fma(g.x, x.x, fma(g.y, x.y, acc[i][j].x));
+          acc[i][j].y = fma(g.y, x.x, fma(-g.x, x.y, acc[i][j].y));
+        }
+    }
   }
   // workgroup reduction: shuffles inside each wavefront, then LDS
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -540,11 +551,11 @@ template <typename T>
 void launch_op_grad(int rb, unsigned grid, T* grad, double* part, const T* g, const T* x,
                     const OpGradParams& p, hipStream_t s) {
   if (rb == 1)
-    op_grad_partial_kernel<T, 1, 1><<<grid, kBlock, 0, s>>>(grad, part, g, x, p);
+    op_grad_partial_kernel<T, 1, 1, 4><<<grid, kBlock, 0, s>>>(grad, part, g, x, p);
   else if (rb == 2)
-    op_grad_partial_kernel<T, 2, 2><<<grid, kBlock, 0, s>>>(grad, part, g, x, p);
+    op_grad_partial_kernel<T, 2, 2, 4><<<grid, kBlock, 0, s>>>(grad, part, g, x, p);
   else
-    op_grad_partial_kernel<T, 4, 4><<<grid, kBlock, 0, s>>>(grad, part, g, x, p);
+    op_grad_partial_kernel<T, 4, 4, 2><<<grid, kBlock, 0, s>>>(grad, part, g, x, p);
 }
 
 }  // namespace
