@@ -169,6 +169,12 @@ int ua_apply_controlled_flip(void* out, const void* state, int dtype, int num_qu
  * bytes, 8-byte aligned.  One state.
  */
 int64_t ua_sample_workspace_bytes(int num_qubits, int64_t num_samples);
+/* Byte offset in that workspace of the float64 total sum |state|^2 once
+ * ua_sample has run on the stream (-1 for invalid sizes).  A total that is
+ * zero, negative or not finite is the case in which Categorical(probs)
+ * raises ValueError (measurement.py:45-48); the draws are then meaningless
+ * and the caller reports the error. */
+int64_t ua_sample_total_offset(int num_qubits, int64_t num_samples);
 int ua_sample(int64_t* out, const void* state, int dtype, int num_qubits, int64_t num_samples,
               uint64_t seed, uint64_t offset, void* workspace, ua_stream_t stream);
 

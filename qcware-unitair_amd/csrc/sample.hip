@@ -322,6 +322,14 @@ extern "C" int64_t ua_sample_workspace_bytes(int num_qubits, int64_t num_samples
   return ua::sample_layout(num_qubits, num_samples).bytes;
 }
 
+extern "C" int64_t ua_sample_total_offset(int num_qubits, int64_t num_samples) {
+  if (num_qubits < 0 || num_qubits > 40 || num_samples < 0 || num_samples > 0x7fffffffLL)
+    return -1;
+  // chunk_sum[chunks], then prefix[chunks]: the total is the last prefix
+  const int64_t chunks = ua::sample_layout(num_qubits, num_samples).chunks;
+  return 8 * (2 * chunks - 1);
+}
+
 extern "C" int ua_sample(int64_t* out, const void* state, int dtype, int num_qubits,
                          int64_t num_samples, uint64_t seed, uint64_t offset, void* workspace,
                          ua_stream_t stream) {

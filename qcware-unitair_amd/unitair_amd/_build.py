@@ -54,13 +54,28 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every HIP source and link the shared library; returns its path."""
+def _stale(src: Path) -> bool:
+    """The object is missing or older than its source or any csrc header."""
+    obj = BUILD_DIR / (src.stem + ".o")
+    if not obj.exists():
+        return True
+    newest = max([src.stat().st_mtime] + [h.stat().st_mtime for h in CSRC_DIR.glob("*.h")] +
+                 [(PROJECT_DIR.parent / "include" / "unitair_amd.h").stat().st_mtime])
+    return obj.stat().st_mtime < newest
+
+
+def build(verbose: bool = False, changed_only: bool = False) -> Path:
+    """Compile every HIP source (with changed_only, only those whose object is
+    out of date) and link the shared library; returns its path."""
     BUILD_DIR.mkdir(exist_ok=True)
     srcs = _sources()
+    objs_all = [BUILD_DIR / (src.stem + ".o") for src in srcs]
+    if changed_only:
+        srcs = [src for src in srcs if _stale(src)]
     workers = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
-    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
-        objs = list(ex.map(_compile, srcs))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        list(ex.map(_compile, srcs))
+    objs = objs_all
     tmp = LIB_PATH.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC",
            *map(str, objs), "-o", str(tmp)]
@@ -74,4 +89,4 @@ def build(verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, changed_only="--changed" in sys.argv)

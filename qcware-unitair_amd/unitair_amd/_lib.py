@@ -55,6 +55,7 @@ SIGNATURES = {
     "ua_rand_state": (_i32, [_p, _i32, _i32, _i64, _u64, _p, _p]),
     "ua_fill_state": (_i32, [_p, _i32, _i32, _i64, _i32, _i64, _p]),
     "ua_sample_workspace_bytes": (_i64, [_i32, _i64]),
+    "ua_sample_total_offset": (_i64, [_i32, _i64]),
     "ua_sample": (_i32, [_p, _p, _i32, _i32, _i64, _u64, _u64, _p, _p]),
     "ua_permute_qubits": (_i32, [_p, _p, _i32, _i32, _pi32, _i64, _i64, _p]),
     "ua_apply_sign_gates": (_i32, [_p, _p, _i32, _i32, _i32, _pi32, _i32, _u64, _i64, _i64, _p]),

@@ -129,6 +129,14 @@ def _segment_stage(ops: List[_Item], cube_state_bits) -> list:
     return out
 
 
+def _check_wide(k: int):
+    """Dense gates wider than 2 qubits run in place through ua_apply_operator,
+    which updates in place for 1..UA_MAX_GATE_QUBITS target qubits."""
+    if not 1 <= k <= _lib.UA_MAX_GATE_QUBITS:
+        raise ValueError(f"fused programs take dense gates on 1..{_lib.UA_MAX_GATE_QUBITS} "
+                         f"qubits (got {k})")
+
+
 def _items_from_circuit(circ: Circuit) -> List[_Item]:
     n = circ.num_qubits
     items = []
@@ -136,6 +144,7 @@ def _items_from_circuit(circ: Circuit) -> List[_Item]:
         bits = tuple(n - 1 - q for q in g.qubits)
         if g.kind == "dense":
             k = len(g.qubits)
+            _check_wide(k)
             items.append(_Item(f"dense{k}" if k <= 2 else "wide", bits,
                                np.asarray(g.matrix, dtype=np.complex128)))
         elif g.kind == "phase":
@@ -243,6 +252,9 @@ class FusedProgram:
     def _build(self, items, n, dtype, device, tile_bits):
         if dtype not in (torch.complex64, torch.complex128):
             raise ValueError("fused programs run complex64/complex128 states")
+        for it in items:
+            if it.kind == "wide":
+                _check_wide(len(it.bits))
         self.lib = _lib.load()
         self.n = n
         if self.n < CUBE_BITS:

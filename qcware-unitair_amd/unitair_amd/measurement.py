@@ -51,6 +51,15 @@ def sample_indices(state: torch.Tensor, num_samples: int,
                            num_samples, seed, 0, _lib.ptr(ws),
                            _lib.stream_handle(state.device))
     _lib.check(rc, "measure")
+    # Categorical(probs) (measurement.py:45-48) rejects a distribution whose
+    # total is zero or not finite; the draws above are then meaningless
+    off = lib.ua_sample_total_offset(n, num_samples)
+    total = ws[off:off + 8].view(torch.float64)
+    if not bool(((total > 0) & torch.isfinite(total)).item()):
+        raise ValueError(
+            f"Expected parameter probs (Tensor of shape {tuple(state.shape)}) of distribution "
+            f"Categorical to satisfy the constraint Simplex(), but found invalid values: "
+            f"sum of |state|^2 = {float(total.item())}")
     return out
 
 

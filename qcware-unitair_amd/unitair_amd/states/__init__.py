@@ -188,10 +188,87 @@ def _as_supported(t: torch.Tensor, what: str) -> torch.Tensor:
     return t
 
 
-def abs_squared(state: torch.Tensor) -> torch.Tensor:
-    """|x|^2 elementwise, real dtype of the input (innerprod.py:4-26)."""
-    require_gpu(state, "abs_squared")
-    state = _as_supported(state, "abs_squared").contiguous()
+def _needs_grad(*ts) -> bool:
+    return torch.is_grad_enabled() and any(isinstance(t, torch.Tensor) and t.requires_grad
+                                           for t in ts)
+
+
+def _real_scale(g: torch.Tensor, like: torch.Tensor) -> torch.Tensor:
+    """A real gradient factor in the real dtype of `like`."""
+    return g.to(_REAL_OF.get(like.dtype, like.dtype) if not g.is_complex() else g.dtype)
+
+
+def _grad_to(x: torch.Tensor, full: torch.Tensor) -> torch.Tensor:
+    """Reduce a full-shape gradient to x's shape and dtype (real inputs keep
+    the real part, as torch does for a real leaf)."""
+    full = full.sum_to_size(x.shape)
+    if not x.is_complex() and full.is_complex():
+        full = full.real
+    return full.to(x.dtype)
+
+
+class _AbsSquared(torch.autograd.Function):
+    """|x|^2 elementwise (innerprod.py:4-26).  Backward: grad_x = 2 g x."""
+
+    @staticmethod
+    def forward(ctx, state):
+        ctx.save_for_backward(state)
+        return _abs_squared_launch(state)
+
+    @staticmethod
+    def backward(ctx, g):
+        (state,) = ctx.saved_tensors
+        return _grad_to(state, 2 * _real_scale(g, state) * state)
+
+
+class _Reduce(torch.autograd.Function):
+    """A row reduction on the HIP kernels with a torch-convention backward
+    (conjugate Wirtinger for complex inputs), the same gradients as torch's
+    autograd through the reference's expressions in innerprod.py:29-65:
+      norm  out = sum |s|^2                 grad_s = 2 g s
+      inner out = sum conj(a) b             grad_a = conj(g) b, grad_b = g a
+      diag  out = sum |s|^2 d  (a = d)      grad_d = g |s|^2,
+                                            grad_s = 2 Re(conj(d) g) s
+    Broadcast inputs get their gradients summed back to their own shapes."""
+
+    @staticmethod
+    def forward(ctx, kind, out_dtype, a, b):
+        ctx.kind = kind
+        ctx.save_for_backward(a, b)
+        return _reduce(kind, out_dtype, a, b)
+
+    @staticmethod
+    def backward(ctx, g):
+        a, b = ctx.saved_tensors
+        g = g.unsqueeze(-1)
+        ga = gb = None
+        if ctx.kind == "norm":
+            if ctx.needs_input_grad[2]:
+                ga = _grad_to(a, 2 * _real_scale(g, a) * a)
+        elif ctx.kind == "inner":
+            if ctx.needs_input_grad[2]:
+                ga = _grad_to(a, (g.conj() if g.is_complex() else g) * b)
+            if ctx.needs_input_grad[3]:
+                gb = _grad_to(b, g * a)
+        else:
+            d, s = a, b
+            if ctx.needs_input_grad[2]:
+                p = (s.conj() * s).real if s.is_complex() else s * s
+                ga = _grad_to(d, g * p.to(g.real.dtype if g.is_complex() else g.dtype))
+            if ctx.needs_input_grad[3]:
+                gp = (d.conj() * g).real if (d.is_complex() or g.is_complex()) else d * g
+                gb = _grad_to(s, 2 * gp.to(_REAL_OF.get(s.dtype, s.dtype)) * s)
+        return None, None, ga, gb
+
+
+def _reduce_op(kind: str, out_dtype: torch.dtype, a: torch.Tensor, b: torch.Tensor = None):
+    if _needs_grad(a, b):
+        return _Reduce.apply(kind, out_dtype, a, b)
+    return _reduce(kind, out_dtype, a, b)
+
+
+def _abs_squared_launch(state: torch.Tensor) -> torch.Tensor:
+    state = state.contiguous()
     out = torch.empty(state.shape, dtype=_REAL_OF[state.dtype], device=state.device)
     if state.numel():
         lib = _lib.load()
@@ -201,18 +278,27 @@ def abs_squared(state: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def abs_squared(state: torch.Tensor) -> torch.Tensor:
+    """|x|^2 elementwise, real dtype of the input (innerprod.py:4-26)."""
+    require_gpu(state, "abs_squared")
+    state = _as_supported(state, "abs_squared")
+    if _needs_grad(state):
+        return _AbsSquared.apply(state)
+    return _abs_squared_launch(state)
+
+
 def norm_squared(state: torch.Tensor) -> torch.Tensor:
     """<state|state> per batch entry (innerprod.py:29-46), float64 accumulation,
     result in the real dtype of the state."""
     require_gpu(state, "norm_squared")
     state = _as_supported(state, "norm_squared")
-    return _reduce("norm", _REAL_OF[state.dtype], state)
+    return _reduce_op("norm", _REAL_OF[state.dtype], state)
 
 
 def norm_squared_f64(state: torch.Tensor) -> torch.Tensor:
     """norm_squared with a float64 result."""
     require_gpu(state, "norm_squared_f64")
-    return _reduce("norm", torch.float64, _as_supported(state, "norm_squared_f64"))
+    return _reduce_op("norm", torch.float64, _as_supported(state, "norm_squared_f64"))
 
 
 def inner_product(state_1: torch.Tensor, state_2: torch.Tensor) -> torch.Tensor:
@@ -223,7 +309,7 @@ def inner_product(state_1: torch.Tensor, state_2: torch.Tensor) -> torch.Tensor:
     b = _as_supported(state_2, "inner_product")
     dt = torch.promote_types(a.dtype, b.dtype)
     a, b = a.to(dt), b.to(dt)
-    return _reduce("inner", dt, a, b)
+    return _reduce_op("inner", dt, a, b)
 
 
 def inner_product_f64(state_1: torch.Tensor, state_2: torch.Tensor) -> torch.Tensor:
@@ -235,8 +321,8 @@ def inner_product_f64(state_1: torch.Tensor, state_2: torch.Tensor) -> torch.Ten
     dt = torch.promote_types(a.dtype, b.dtype)
     a, b = a.to(dt), b.to(dt)
     if not dt.is_complex:
-        return _reduce("inner", torch.float64, a, b)
-    return _reduce("inner", torch.complex128, a, b)
+        return _reduce_op("inner", torch.float64, a, b)
+    return _reduce_op("inner", torch.complex128, a, b)
 
 
 def diag_expectation_value(diag_values: torch.Tensor, state: torch.Tensor) -> torch.Tensor:
@@ -251,7 +337,7 @@ def diag_expectation_value(diag_values: torch.Tensor, state: torch.Tensor) -> to
     d = _as_supported(diag_values, "diag_expectation_value")
     if out_dtype not in _lib.DTYPE_CODES:
         raise TypeError(f"diag_expectation_value: unsupported result dtype {out_dtype}")
-    return _reduce("diag", out_dtype, d, s)
+    return _reduce_op("diag", out_dtype, d, s)
 
 
 # submodules with the reference's layout (unitair/states/{shapes,conversions,innerprod}.py)

@@ -101,6 +101,9 @@ def test_validation_errors_of_layer_permutation_sampling_and_constructors(lib):
     assert lib.ua_sample(None, a, c64, 41, 5, 0, 0, b, None) == bad
     assert lib.ua_sample(None, a, c64, 4, 1 << 31, 0, 0, b, None) == bad
     assert lib.ua_sample_workspace_bytes(41, 5) == -1
+    assert lib.ua_sample_total_offset(41, 5) == -1
+    assert lib.ua_sample_total_offset(12, 0) == 8     # one chunk: prefix[0] after chunk_sum[0]
+    assert lib.ua_sample_total_offset(20, 7) == 8 * (2 * 256 - 1)
     assert lib.ua_sample_workspace_bytes(12, 1 << 31) == -1
     # chunk total and prefix (float64), 8 bucket sub-counts and a bucket
     # start (uint32) per 4096-amplitude chunk; chunk key, slot and bucket

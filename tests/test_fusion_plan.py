@@ -144,3 +144,19 @@ def test_stage_segments_keep_the_stage_result():
         flat.extend([g for lane in x for g in lane] if k == "layer" else [x])
     psi = rng.standard_normal(2 ** n) + 1j * rng.standard_normal(2 ** n)
     assert np.allclose(apply(flat, psi), apply(items, psi), atol=1e-13)
+
+
+def test_dense_gates_wider_than_the_in_place_limit_are_rejected_at_build():
+    """Wide dense steps run in place through ua_apply_operator (1..5 target
+    qubits); a 6-qubit gate is refused while the program is built, before
+    any device buffer exists."""
+    import torch
+    from unitair_amd.fusion import FusedProgram
+    rng = np.random.default_rng(0)
+    circ = Circuit(8, [Gate("dense", (0,), matrix=haar_unitary(2, rng)),
+                       Gate("dense", tuple(range(6)), matrix=haar_unitary(64, rng))])
+    with pytest.raises(ValueError, match="1..5"):
+        FusedProgram(circ, dtype=torch.complex64, device="cpu")
+    items = [fusion._Item("wide", tuple(range(6)), haar_unitary(64, rng))]
+    with pytest.raises(ValueError, match="1..5"):
+        FusedProgram.from_items(items, 8, dtype=torch.complex64, device="cpu")

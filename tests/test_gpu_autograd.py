@@ -197,3 +197,57 @@ def test_adjoint_16q_batch1024_c64(device):
             assert abs(fd - ps) < 1e-5
             pz = AO.param_shift(th[b], ph[b], n, "phi", l, q)
             assert abs(gp128[b, l, q] - pz) < 1e-9
+
+
+# Reductions are differentiable like the reference's torch expressions
+# (innerprod.py:4-65): a VQE-style loss built from the drop-in calls only.
+@pytest.mark.parametrize("dt", [torch.complex128, torch.float64])
+def test_reduction_gradients(device, dt):
+    import unitair_amd as ua
+    torch.manual_seed(11)
+    a = torch.randn(3, 2 ** 5, dtype=dt)
+    b = torch.randn(3, 2 ** 5, dtype=dt)
+    d = torch.randn(2 ** 5, dtype=torch.float64)
+    dc = torch.randn(2, 1, 2 ** 5, dtype=torch.complex128)
+    cases = [
+        (lambda x: T.norm_squared(x), lambda x: ua.norm_squared(x), (a,)),
+        (lambda x: T.abs_squared(x), lambda x: ua.abs_squared(x), (a,)),
+        (lambda x, y: T.inner_product(x, y), lambda x, y: ua.inner_product(x, y), (a, b)),
+        (lambda x, y: T.inner_product(x, y), lambda x, y: ua.states.inner_product_f64(x, y),
+         (a, b[:1])),
+        (lambda h, x: T.diag_expectation_value(h, x),
+         lambda h, x: ua.diag_expectation_value(h, x), (d, a)),
+        (lambda h, x: T.diag_expectation_value(h, x),
+         lambda h, x: ua.diag_expectation_value(h, x), (dc, a)),
+    ]
+    for ref_fn, fn, xs in cases:
+        ref = _grads_cpu(ref_fn, *xs)
+        got = _grads_gpu(fn, device, *xs)
+        for r, g in zip(ref, got):
+            assert r.shape == g.shape and r.dtype == g.dtype
+            assert torch.allclose(r, g, atol=1e-10), fn
+
+
+def test_vqe_style_loss_through_drop_in_calls(device):
+    """diag_expectation_value(H, apply_operator(U(theta), q, psi)) with the
+    gradient reaching theta, against the same expression on the CPU
+    restatement (complex128)."""
+    import unitair_amd as ua
+    n = 6
+    torch.manual_seed(5)
+    psi = torch.randn(2, 2 ** n, dtype=torch.complex128)
+    h = torch.randn(2 ** n, dtype=torch.float64)
+    theta = torch.tensor([0.3, -1.1], dtype=torch.float64)
+
+    def loss(apply, expect, t, st, hh):
+        c, s = torch.cos(t / 2), torch.sin(t / 2)
+        u = torch.stack([torch.stack([c, -s], -1), torch.stack([s, c], -1)], -2)
+        out = apply(u.to(st.dtype), [2], st)
+        return expect(hh, out).sum()
+
+    tr = theta.clone().requires_grad_(True)
+    loss(T.apply_operator, T.diag_expectation_value, tr, psi, h).backward()
+    tg = theta.to(device).requires_grad_(True)
+    loss(sim.apply_operator, ua.diag_expectation_value, tg, psi.to(device),
+         h.to(device)).backward()
+    assert torch.allclose(tg.grad.cpu(), tr.grad, atol=1e-11)

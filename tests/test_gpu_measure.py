@@ -128,3 +128,21 @@ def test_uniform_weights_25_qubits_global_prefix_search(device):
     got = sample_indices(state, k, generator=torch.Generator().manual_seed(21)).cpu().numpy()
     seed = _seed(21)
     assert np.array_equal(got, [int(uniform53(s, seed) * 2 ** n) for s in range(k)])
+
+
+@pytest.mark.parametrize("bad", ["zero", "nan", "inf"])
+@pytest.mark.parametrize("n", [4, 13])
+def test_invalid_distribution_raises_value_error(device, bad, n):
+    """Categorical(probs) raises ValueError when the probabilities sum to
+    zero or are not finite (measurement.py:45-48); so do measure and
+    sample_indices, for a zero, a NaN and an infinite amplitude."""
+    state = torch.zeros(2 ** n, dtype=torch.complex64, device=device)
+    if bad == "nan":
+        state[3] = float("nan")
+    elif bad == "inf":
+        state[:] = 1
+        state[5] = float("inf")
+    with pytest.raises(ValueError):
+        sample_indices(state, 100)
+    with pytest.raises(ValueError):
+        sim.measure(state, 10)
