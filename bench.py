@@ -45,6 +45,10 @@ def parse():
     p.add_argument("--no-stream-ref", action="store_true",
                    help="skip timing torch's own streaming kernels on the same state size")
     p.add_argument("--no-fused", action="store_true", help="skip the fused-pass leg")
+    p.add_argument("--no-28q", action="store_true",
+                   help="skip the 28-qubit (beyond the Infinity Cache) per-gate and fused leg")
+    p.add_argument("--latency-runs", type=int, default=100,
+                   help="single synchronised circuit runs for the p50/p99 latency")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="run one float32 multiply-by-one of this many MiB (read + write) "
                         "before the timed steps, as the known-size kernel that scales "
@@ -81,6 +85,98 @@ def time_steps(runner, state, steps, warmup, with_events):
             for kind, ms in zip(runner.kinds, pool.elapsed_ms()):
                 per_kind.setdefault(kind, []).append(ms)
     return (t1 - t0) / steps, per_kind
+
+
+def step_bytes_of(circ, amps, es=8, angle_es=4):
+    """Algorithmic bytes of one pass of the circuit: one read and one write of
+    the state per dense gate and phase (plus the phase's angles); in-place CZ
+    and CNOT count only the amplitudes they touch (a quarter and a half)."""
+    total = 0
+    for g in circ.gates:
+        full = 2 * amps * es
+        if g.kind == "cz":
+            total += full // 4
+        elif g.kind == "cnot":
+            total += full // 2
+        elif g.kind == "phase":
+            total += full + amps * angle_es
+        else:
+            total += full
+    return total
+
+
+def step_latencies(runner, state, steps):
+    """Latency of single circuit runs (synchronised on both sides):
+    p50 / p99 / max in ms over `steps` runs."""
+    import numpy as np
+    import torch
+    runner.run(state)
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        runner.run(state)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    lat = np.asarray(lat)
+    return {"runs": steps, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+            "mean_ms": float(lat.mean())}
+
+
+def device_angles(circ, device, seed):
+    """The circuit with every phase gate's angles drawn on the device
+    (uniform [0, 2pi) float32, the same distribution as the seeded host
+    angles): large states then need no host generation or upload."""
+    import torch
+    from unitair_amd.circuits import Circuit, Gate
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    gates = []
+    for g in circ.gates:
+        if g.kind == "phase":
+            a = torch.rand(2 ** circ.num_qubits, generator=gen, device=device,
+                           dtype=torch.float32) * (2 * 3.141592653589793)
+            gates.append(Gate("phase", (), angles=a))
+        else:
+            gates.append(g)
+    return Circuit(circ.num_qubits, gates)
+
+
+def per_gate_28q(dev, seed, n=28, steps=3, warmup=1):
+    """The same circuit recipe on a 28-qubit state (2 GiB, larger than the
+    256 MB Infinity Cache, so every pass goes to HBM): per-gate launches in
+    place, and the fused passes."""
+    import numpy as np
+    import torch
+    from unitair_amd.circuits import CircuitRunner
+    from unitair_amd.fusion import FusedProgram
+    from unitair_amd.initializations import rand_state
+    from unitair_amd.workloads import random_circuit_24q
+    circ = device_angles(random_circuit_24q(seed=seed, n=n), dev, seed + 100)
+    amps = 2 ** n
+    state = rand_state(n, device=dev, dtype=torch.complex64, seed=seed + 10)
+    runner = CircuitRunner(circ, dtype=torch.complex64, device=dev, in_place=True)
+    sec, per_kind = time_steps(runner, state, steps, warmup, with_events=True)
+    nbytes = step_bytes_of(circ, amps)
+    dense1 = 1e3 * float(np.mean(per_kind["dense1q"]))
+    out = {"workload": f"random circuit, {n} qubits, {len(circ.gates)} gates "
+                       f"({circ.counts()}), one complex64 state (2 GiB)",
+           "ms_per_step": sec * 1e3, "amp_updates_per_s": len(circ.gates) * amps / sec,
+           "step_algorithmic_bytes": nbytes, "step_algorithmic_GBps": nbytes / sec / 1e9,
+           "step_frac_of_hbm_peak": nbytes / sec / 1e9 / HBM_PEAK_GBS,
+           "dense1q_avg_us": dense1,
+           "dense1q_frac_of_hbm_peak": 2 * amps * 8 / dense1 / 1e3 / HBM_PEAK_GBS}
+    del runner
+    prog = FusedProgram(circ, dtype=torch.complex64, device=dev)
+    sec_f, _ = time_steps(prog, state, steps, warmup, with_events=False)
+    out["fused"] = {"passes": prog.num_passes, "ms_per_step": sec_f * 1e3,
+                    "amp_updates_per_s": len(circ.gates) * amps / sec_f,
+                    "ms_per_pass": sec_f * 1e3 / prog.num_passes,
+                    "pass_algorithmic_GBps": 2 * amps * 8 * prog.num_passes / sec_f / 1e9}
+    del prog, state, circ
+    torch.cuda.empty_cache()
+    return out
 
 
 def time_graph(runner, state, steps, warmup):
@@ -538,6 +634,8 @@ def main():
         + ", ".join(f"{k}={1e3 * sum(v) / len(v):.1f}" for k, v in per_kind.items()))
 
     extra = {}
+    log(f"latency of single runs ({args.latency_runs} runs)")
+    extra["latency_per_gate"] = step_latencies(runner, state, args.latency_runs)
     if not args.no_out_of_place:
         log("timing out-of-place form")
         r2 = CircuitRunner(circ, dtype=torch.complex64, device=dev, in_place=False)
@@ -565,7 +663,9 @@ def main():
                           "ms_per_step": sec4 * 1e3,
                           "amp_updates_per_s": ngates * amps / sec4,
                           "graph_ms_per_step": sec5 * 1e3,
-                          "graph_amp_updates_per_s": ngates * amps / sec5}
+                          "graph_amp_updates_per_s": ngates * amps / sec5,
+                          "ms_per_pass": sec4 * 1e3 / prog.num_passes,
+                          "latency": step_latencies(prog, state, args.latency_runs)}
         log(f"fused: {prog.num_passes} passes, {sec4 * 1e3:.3f} ms/step eager, "
             f"{sec5 * 1e3:.3f} ms/step graph")
         del prog
@@ -579,12 +679,14 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
                 "traffic": measured_traffic() if n == 24 else None,
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this "
+                                  "command, scaled by a copy kernel of known size "
+                                  "(profiles/traffic_24q_circuit.md); not measured in this run",
                 "kernel": "dense_fast_kernel<complex64,k=1>",
                 "bytes_per_launch": dom_bytes}
     # whole-step algorithmic bandwidth (the metric's "HBM GB/s")
-    step_bytes = 0
-    for g in circ.gates:
-        step_bytes += 2 * amps * 8 + (amps * 4 if g.kind == "phase" else 0)
+    step_bytes = step_bytes_of(circ, amps)
+    extra["step_algorithmic_bytes"] = step_bytes
     extra["step_algorithmic_GBps"] = step_bytes / sec / 1e9
     extra["step_frac_of_hbm_peak"] = extra["step_algorithmic_GBps"] / HBM_PEAK_GBS
     extra["kernels"] = kernels
@@ -609,6 +711,15 @@ def main():
         extra["stream_reference"] = stream_reference(dev, n)
         log("stream reference: " + ", ".join(f"{k} {v['GBps']:.0f} GB/s"
                                               for k, v in extra["stream_reference"].items()))
+
+    if not args.no_28q and n < 28:
+        log("28-qubit leg (per-gate and fused)")
+        extra["circuit_28q"] = per_gate_28q(dev, args.seed)
+        c28 = extra["circuit_28q"]
+        log(f"28q: per-gate {c28['ms_per_step']:.1f} ms/step "
+            f"({c28['step_frac_of_hbm_peak']:.3f} of HBM peak), dense1q "
+            f"{c28['dense1q_avg_us']:.0f} us; fused {c28['fused']['passes']} passes "
+            f"{c28['fused']['ms_per_step']:.2f} ms/step")
 
     if not args.no_fused_dense:
         log("fused dense 4/5-qubit blocks, 28 qubits")

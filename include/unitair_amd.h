@@ -308,9 +308,13 @@ int ua_run_circuit(void* state, void* work, int dtype, int num_qubits, int64_t b
  * tile of the state (m = tile_bits <= 12 "local" index bits) while it sits in
  * LDS.  In place, complex64 or complex128.
  *
- * A pass is cut into stages.  In a stage each thread holds the 16 amplitudes
- * spanned by the stage's 4 tile bits (its "cube"); dense gates address cube
- * bits 0..3, element-wise gates address state index bits.
+ * A pass is cut into stages of two kinds.  In a cube stage each thread holds
+ * the 16 amplitudes spanned by the stage's 4 tile bits (its "cube"); dense
+ * gates address cube bits 0..3, element-wise gates address state index bits.
+ * A diagonal stage (stage_bits {-1, -1, -1, -1}) holds only CZ, DIAG1, PHASE
+ * and CZ_LAYER ops and visits the amplitudes in the order of the state's
+ * coalesced load, so per-element data (phase angles) is read contiguously.
+ * PHASE ops belong in diagonal stages.
  */
 #define UA_FUSED_MAX_STAGES 32
 
@@ -332,33 +336,32 @@ enum {
                         applied grouped by bit with no per-op dispatch */
 };
 
-#define UA_FUSED_MATRIX_CACHE 1024 /* elements of gate data a pass keeps on chip */
-
 typedef struct {
   int32_t kind; /* UA_FOP_* */
   int32_t a, b;
-  int32_t cache_slot;   /* -1, or element offset of this op's data in the pass's
-                           on-chip cache (data_bstride must be 0; the op's 4, 16
-                           or 2 elements must end within UA_FUSED_MATRIX_CACHE) */
-  const void* data;     /* device pointer */
-  int64_t data_bstride; /* elements between the data of consecutive states */
+  int32_t cache_slot;   /* unused (kept for the record layout); set -1 */
+  const void* data;     /* device pointer; read-only while the pass runs (gate data
+                           is read through scalar loads) */
+  int64_t data_bstride; /* elements between the data of consecutive states; PHASE
+                           angles must be aligned to two angles and data_bstride
+                           even */
 } ua_fused_op;
 
 typedef struct {
   int32_t tile_bits;           /* m in 4..12, m <= num_qubits */
   int32_t local_bits[12];      /* state bit of tile bit i: ascending, [0..4) = 0,1,2,3 */
   int32_t num_stages;          /* <= UA_FUSED_MAX_STAGES */
-  int32_t needs_index;         /* 1 if any op is CNOT/CZ/DIAG1/PHASE/CZ_LAYER */
-  int32_t stage_bits[UA_FUSED_MAX_STAGES][4]; /* ascending tile bits of each cube */
+  int32_t needs_index;         /* unused (derived from the ops); kept for the layout */
+  int32_t stage_bits[UA_FUSED_MAX_STAGES][4]; /* ascending tile bits of each cube, or
+                                                 all -1 for a diagonal stage */
   int32_t stage_ops[UA_FUSED_MAX_STAGES + 1]; /* stage s runs ops [stage_ops[s], stage_ops[s+1]) */
 } ua_fused_pass;
 
 /* Run one fused pass on `batch` contiguous states.  `ops` is the device copy
- * of the pass's ops; `host_ops` (may be NULL) is the same array in host
- * memory: it is validated before launch, and when it holds no PHASE and no
- * DENSE2 op a leaner kernel without those two op kinds runs the pass (host_ops
- * must then be an exact copy of ops).  With host_ops NULL the full kernel
- * runs. */
+ * of the pass's ops; `host_ops` is the same array in host memory (required
+ * when num_stages > 0): it is validated before launch and selects the kernel
+ * variant (1-qubit layers only / no DENSE2 / every kind), so it must be an
+ * exact copy of ops. */
 int ua_run_fused_pass(void* state, int dtype, int num_qubits, int64_t batch,
                       const ua_fused_pass* pass, const ua_fused_op* ops,
                       const ua_fused_op* host_ops, ua_stream_t stream);

@@ -116,7 +116,6 @@ class FusedPass(ctypes.Structure):
                 ("stage_ops", ctypes.c_int32 * (UA_FUSED_MAX_STAGES + 1))]
 
 
-UA_FUSED_MATRIX_CACHE = 1024
 SIGNATURES["ua_run_fused_pass"] = (_i32, [_p, _i32, _i32, _i64, ctypes.POINTER(FusedPass),
                                           _p, _p, _p])
 SIGNATURES["ua_run_circuit"] = (_i32, [_p, _p, _i32, _i32, _i64, ctypes.POINTER(Gate), _i32,

@@ -115,7 +115,10 @@ class CircuitRunner:
                 offset += 4 ** len(g.qubits)
                 kinds.append(f"dense{len(g.qubits)}q")
             elif g.kind == "phase":
-                a = torch.as_tensor(np.ascontiguousarray(g.angles)).to(self.device, real)
+                if isinstance(g.angles, torch.Tensor):
+                    a = g.angles.to(self.device, real).contiguous()
+                else:
+                    a = torch.as_tensor(np.ascontiguousarray(g.angles)).to(self.device, real)
                 self._buffers.append(a)
                 c.kind, c.data, c.data_dtype = _lib.UA_GATE_PHASE, a.data_ptr(), \
                     _lib.DTYPE_CODES[real]

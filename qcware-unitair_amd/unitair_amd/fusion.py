@@ -8,8 +8,9 @@ local bits stays within m:
   CNOT                  : needs its target bit local (control: any bit)
   CZ, phase, diagonal   : need nothing (applied element by element)
 Wider dense gates end the pass and run as their own ua_apply_operator call.
-Inside a pass, gates are grouped into stages whose dense targets fit in a
-4-bit cube.  Consecutive 1-qubit gates on the same qubit are multiplied on
+Inside a pass, gates are grouped into stages: cube stages, whose dense
+targets fit in a 4-bit cube, and diagonal stages that carry the full-length
+phases (and diagonal gates next to them) in the coalesced tile order.  Consecutive 1-qubit gates on the same qubit are multiplied on
 the host first (complex128), as unitair's fuse_single_qubit_operators
 (unitair/gates/matrix_algebra.py:6-51) does for apply_to_qubits.
 """
@@ -201,23 +202,47 @@ def plan_passes(items: Sequence[_Item], n: int, tile_bits: int = TILE_BITS):
     return out
 
 
+_DIAG_KINDS = ("phase", "cz", "diag1", "cz_layer")
+
+
 def _stages(p: _Pass):
-    """Split a pass into stages: [(cube tile bits (sorted, 4), [items])]."""
+    """Split a pass into stages: [(cube, [items])].
+
+    cube is the sorted list of 4 tile bits of a cube stage, or None for a
+    diagonal stage.  Full-length phases always run in diagonal stages (the
+    kernel reads their angles in the coalesced tile order); CZ, 1-qubit
+    diagonals and CZ layers that follow a phase join its diagonal stage
+    (diagonal gates commute), otherwise they ride in the current cube stage,
+    which needs no cube bits for them.  Dense gates and CNOT need their
+    target bits in the stage's cube."""
     tile_of = {s: i for i, s in enumerate(p.local)}
     m = len(p.local)
     stages = []
-    cube, ops = set(), []
+    cur = None                       # [cube set or None, items]
     for it in p.items:
+        if it.kind == "phase" or (it.kind in _DIAG_KINDS and cur is not None and
+                                  cur[0] is None):
+            if cur is None or cur[0] is not None:
+                if cur is not None:
+                    stages.append(cur)
+                cur = [None, []]
+            cur[1].append(it)
+            continue
         need = {tile_of[b] for b in _needs(it)}
-        if len(cube | need) <= CUBE_BITS:
-            cube |= need
-            ops.append(it)
-        else:
-            stages.append((cube, ops))
-            cube, ops = set(need), [it]
-    stages.append((cube, ops))
+        if cur is not None and cur[0] is not None and len(cur[0] | need) <= CUBE_BITS:
+            cur[0] |= need
+            cur[1].append(it)
+            continue
+        if cur is not None:
+            stages.append(cur)
+        cur = [set(need), [it]]
+    if cur is not None:
+        stages.append(cur)
     result = []
     for cube, ops in stages:
+        if cube is None:
+            result.append((None, ops))
+            continue
         cube = set(cube)
         for t in range(m):
             if len(cube) >= CUBE_BITS:
@@ -290,10 +315,14 @@ class FusedProgram:
                 tile_of = {s: i for i, s in enumerate(p.local)}
                 for si, (cube, ops) in enumerate(part):
                     for k in range(CUBE_BITS):
-                        fp.stage_bits[si][k] = cube[k]
+                        fp.stage_bits[si][k] = cube[k] if cube is not None else -1
                     fp.stage_ops[si] = len(op_structs) - first_op
-                    pos = {t: i for i, t in enumerate(cube)}
-                    entries = _segment_stage(ops, [p.local[t] for t in cube])
+                    if cube is None:
+                        pos = {}
+                        entries = [("op", it) for it in ops]
+                    else:
+                        pos = {t: i for i, t in enumerate(cube)}
+                        entries = _segment_stage(ops, [p.local[t] for t in cube])
                     flat_ops = []
                     for kind, x in entries:
                         if kind == "op":
@@ -339,16 +368,18 @@ class FusedProgram:
                 fp.needs_index = needs_index
                 self.steps.append(("fused", fp, first_op, len(op_structs) - first_op))
         for o in op_structs:
-            o.cache_slot = -1
+            o.cache_slot = -1           # unused by the kernel (gate data is read as scalars)
         # upload operand data and fill data pointers
         for idx, it in pending:
             o = op_structs[idx]
             want = {"phase": real, "cz_layer": torch.int64}.get(it.kind, dtype)
-            if isinstance(it.data, torch.Tensor):
+            if isinstance(it.data, torch.Tensor) and it.per_state:
                 t = it.data
                 if t.device != self.device or t.dtype != want or not t.is_contiguous():
                     raise ValueError("device gate data must be contiguous, of the program's "
                                      "dtype and device")
+            elif isinstance(it.data, torch.Tensor):
+                t = it.data.to(self.device, want).contiguous()
             else:
                 t = torch.as_tensor(np.ascontiguousarray(it.data)).to(self.device, want)
             per = {"phase": 2 ** self.n, "diag1": 2, "dense1": 4, "dense2": 16,
@@ -356,18 +387,6 @@ class FusedProgram:
             o.data_bstride = per if it.per_state else 0
             self._keep.append(t)
             o.data = t.data_ptr()
-        # on-chip cache slots for data shared by every state, per pass
-        for st in self.steps:
-            if st[0] != "fused":
-                continue
-            _, fp, first, count = st
-            slot = 0
-            for o in op_structs[first:first + count]:
-                if o.kind in (FOP_DENSE1, FOP_DENSE2, FOP_DIAG1) and o.data_bstride == 0:
-                    size = {FOP_DENSE1: 4, FOP_DENSE2: 16, FOP_DIAG1: 2}[o.kind]
-                    if slot + size <= _lib.UA_FUSED_MATRIX_CACHE:
-                        o.cache_slot = slot
-                        slot += size
         arr = (FusedOp * max(1, len(op_structs)))(*op_structs)
         self._host_ops = arr
         raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)

@@ -80,7 +80,13 @@ def test_plan_respects_limits(n, seed):
         stages = fusion._stages(p)
         seen = []
         for cube, ops in stages:
+            if cube is None:            # diagonal stage: diagonal gates only, with a phase
+                assert ops and all(it.kind in ("phase", "cz", "diag1", "cz_layer") for it in ops)
+                assert ops[0].kind == "phase"
+                seen += ops
+                continue
             assert len(cube) == 4 and cube == sorted(cube) and all(0 <= t < m for t in cube)
+            assert all(it.kind != "phase" for it in ops)
             tile_of = {s: i for i, s in enumerate(p.local)}
             for it in ops:
                 assert {tile_of[b] for b in fusion._needs(it)} <= set(cube)
