@@ -311,10 +311,10 @@ int ua_run_circuit(void* state, void* work, int dtype, int num_qubits, int64_t b
  * A pass is cut into stages of two kinds.  In a cube stage each thread holds
  * the 16 amplitudes spanned by the stage's 4 tile bits (its "cube"); dense
  * gates address cube bits 0..3, element-wise gates address state index bits.
- * A diagonal stage (stage_bits {-1, -1, -1, -1}) holds only CZ, DIAG1, PHASE
- * and CZ_LAYER ops and visits the amplitudes in the order of the state's
+ * A diagonal stage (stage_bits {-1, -1, -1, -1}) holds only CZ, DIAG1, PHASE,
+ * PHASE_FACTORS and CZ_LAYER ops and visits the amplitudes in the order of the state's
  * coalesced load, so per-element data (phase angles) is read contiguously.
- * PHASE ops belong in diagonal stages.
+ * PHASE and PHASE_FACTORS ops belong in diagonal stages.
  */
 #define UA_FUSED_MAX_STAGES 32
 
@@ -329,11 +329,14 @@ enum {
   UA_FOP_CZ_LAYER = 6, /* data: uint64 mask(s); bit j set = CZ between index bits
                         j and j+1 (qubits n-2-j and n-1-j); data_bstride in
                         masks (1 = one mask per state) */
-  UA_FOP_LAYER1Q = 7  /* a: four 8-bit counts c0..c3 (c_k in bits 8k..8k+7); the
+  UA_FOP_LAYER1Q = 7, /* a: four 8-bit counts c0..c3 (c_k in bits 8k..8k+7); the
                         next c0+c1+c2+c3 ops are UA_FOP_DENSE1 on cube bit 0
                         (c0 of them, in order), then cube bit 1, 2, 3: 1-qubit
                         gates on distinct bits commute, so a run of them is
                         applied grouped by bit with no per-op dispatch */
+  UA_FOP_PHASE_FACTORS = 8 /* data: 2**n complex factors (state dtype), element x
+                        multiplied by data[x]: apply_phase with exp(-1j*angle)
+                        formed once (aligned to two elements, data_bstride even) */
 };
 
 typedef struct {

@@ -15,9 +15,11 @@
 //   cube stage:     every thread owns a "cube" of 16 amplitudes spanning 4
 //                   tile bits, so the stage's gates run in registers;
 //   diagonal stage: every thread owns the 16 amplitudes it loaded (8 pairs
-//                   in tile order), so per-element data such as phase angles
-//                   is read with the same coalesced pattern as the state.
-// Only stage boundaries go through LDS.
+//                   in tile order), so per-element data such as phase
+//                   factors is read with the same coalesced pattern as the
+//                   state.
+// Only stage boundaries go through LDS; a diagonal stage at the start or the
+// end of a pass runs on the registers of the global load or store.
 //
 // Gate data shared by the workgroup (matrices, angle rows of one state) is
 // read through the constant address space at wave-uniform addresses, i.e.
@@ -205,6 +207,8 @@ struct PassArgs {
   int log2_tiles_per_state; // n - m
   int num_stages;
   int needs_index;          // some cube stage has an op that reads the global index
+  int reg_head, reg_tail;   // the first / last stage is diagonal with PHASE_FACTORS ops
+                            // only: it runs on the registers of the global load / store
   int local[kTileBits];     // state bit of tile bit i (ascending; local[i] = i for i < 4)
   int stage_bits[UA_FUSED_MAX_STAGES][4];  // -1: diagonal stage
   int stage_ops[UA_FUSED_MAX_STAGES + 1];
@@ -245,10 +249,12 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd<T>()))) void fused_pass_kernel
     const uint64_t tis = uint64_t(tg) & ((uint64_t(1) << a.log2_tiles_per_state) - 1);
     return insert_zero_bits(tis, a.local, m);
   };
-  // Pair k of this thread in tile order: block element f = 2*tid + 512*k,
+  // Pair k of this thread in tile order: block element f = 2*t + 512*k,
   // tile sub = f >> m, element e = f & (2^m - 1).  The state index deposits
   // e's bits at the local bit positions: bits 0..3 stay, bits 4..8 come from
-  // tid (dep_t), bits 9.. from k.
+  // t (a per-thread part), bits 9.. from k (a wave-uniform part).  With one
+  // tile per workgroup the LDS position splits the same way:
+  // pos(2t + 512k) = pos(2t) ^ 512k ^ 16(k & 1).
   // `t` is the thread id; callers far apart pass it through opaque() so
   // that the compiler recomputes these few values instead of keeping every
   // pair's address live in registers across the whole pass.
@@ -259,27 +265,39 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd<T>()))) void fused_pass_kernel
     int lds;     // LDS position of the first element (the second is lds ^ 1)
     bool valid;
   };
-  auto pair_at = [&](int t, int k) -> PairAt {
-    uint64_t dep_t = 0;
-    for (int i = 4; i < 9 && i < m; ++i) dep_t |= uint64_t(((2 * t) >> i) & 1) << a.local[i];
-    dep_t |= uint64_t((2 * t) & 15);
-    const int f = 2 * t + 2 * kThreads * k;
-    const int sub = MULTI ? (f >> m) : 0;
-    const int e = MULTI ? (f & ((1 << m) - 1)) : f;
-    uint64_t dep_k = 0;
-    for (int i = 9; i < m; ++i) dep_k |= uint64_t((e >> i) & 1) << a.local[i];
+  struct ThreadPart {
+    uint64_t dep;  // deposit of bits 0..8 of 2t
+    int lds;       // pos(2t)
+  };
+  auto thread_part = [&](int t) -> ThreadPart {
+    ThreadPart tp;
+    tp.dep = uint64_t((2 * t) & 15);
+    for (int i = 4; i < 9 && i < m; ++i) tp.dep |= uint64_t(((2 * t) >> i) & 1) << a.local[i];
+    tp.lds = lds_pos(2 * t);
+    return tp;
+  };
+  auto pair_at = [&](const ThreadPart& tp, int t, int k) -> PairAt {
     PairAt p;
     if constexpr (MULTI) {
+      const int f = 2 * t + 2 * kThreads * k;
+      const int sub = f >> m;
+      const int e = f & ((1 << m) - 1);
+      uint64_t dep_k = 0;
+      for (int i = 9; i < m; ++i) dep_k |= uint64_t((e >> i) & 1) << a.local[i];
       const int64_t tg = (int64_t(blockIdx.x) << tpb_log2) + sub;
       p.valid = tg < a.total_tiles;
-      p.g = (p.valid ? tile_origin(tg) : 0) | dep_k | dep_t;
+      p.g = (p.valid ? tile_origin(tg) : 0) | dep_k | tp.dep;
       p.b = p.valid ? (tg >> a.log2_tiles_per_state) : 0;
+      p.lds = (sub << m) + lds_pos(e);
     } else {
+      uint64_t dep_k = 0;
+#pragma unroll
+      for (int i = 9; i < kTileBits; ++i) dep_k |= uint64_t((k >> (i - 9)) & 1) << a.local[i];
       p.valid = true;
-      p.g = origin1 | dep_k | dep_t;
+      p.g = origin1 | dep_k | tp.dep;
       p.b = int64_t(blockIdx.x) >> a.log2_tiles_per_state;
+      p.lds = tp.lds ^ (2 * kThreads * k) ^ (16 * (k & 1));
     }
-    p.lds = (sub << m) + lds_pos(e);
     return p;
   };
 
@@ -293,76 +311,138 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd<T>()))) void fused_pass_kernel
       return (const UA_SCALAR R*)op.data + 2 * b * op.data_bstride;
   };
 
-  // Diagonal ops in tile order, LDS to LDS, one pair at a time: rolled
-  // loops keep the register footprint of the sincos and index work small.
+  // One diagonal op on one pair (x0, x1) of state indices g, g | 1.
+  auto diag_pair = [&](const ua_fused_op& op, const PairAt& p, T& x0, T& x1) {
+    const uint64_t g0 = p.g, g1 = p.g | 1;
+    switch (op.kind) {
+      case UA_FOP_PHASE_FACTORS: {  // data: 2^n factors per state, state dtype
+        const T* f = static_cast<const T*>(op.data) + p.b * op.data_bstride;
+        const Pair<T> v = *reinterpret_cast<const Pair<T>*>(f + p.g);
+        x0 = cmul(v.v[0], x0);
+        x1 = cmul(v.v[1], x1);
+        break;
+      }
+      case UA_FOP_PHASE: {  // data: 2^n real angles per state, data_bstride in angles
+        const R* th = static_cast<const R*>(op.data) + p.b * op.data_bstride;
+        const Pair<R> t = *reinterpret_cast<const Pair<R>*>(th + p.g);
+        x0 = cmul(expmi(t.v[0]), x0);
+        x1 = cmul(expmi(t.v[1]), x1);
+        break;
+      }
+      case UA_FOP_CZ: {
+        if ((g0 >> op.a) & (g0 >> op.b) & 1) x0 = cneg(x0);
+        if ((g1 >> op.a) & (g1 >> op.b) & 1) x1 = cneg(x1);
+        break;
+      }
+      case UA_FOP_DIAG1: {
+        const R* d = static_cast<const R*>(op.data) + 2 * p.b * op.data_bstride;
+        const T d0 = ldc<T>(d, 0), d1 = ldc<T>(d, 1);
+        x0 = cmul(((g0 >> op.a) & 1) ? d1 : d0, x0);
+        x1 = cmul(((g1 >> op.a) & 1) ? d1 : d0, x1);
+        break;
+      }
+      case UA_FOP_CZ_LAYER: {
+        const uint64_t mask = static_cast<const uint64_t*>(op.data)[p.b * op.data_bstride];
+        if (__popcll(g0 & (g0 >> 1) & mask) & 1) x0 = cneg(x0);
+        if (__popcll(g1 & (g1 >> 1) & mask) & 1) x1 = cneg(x1);
+        break;
+      }
+      default:
+        break;
+    }
+  };
+
+  // Diagonal stage in tile order, LDS to LDS, one pair at a time: rolled
+  // loops keep the register footprint of sincos and index work small.
   auto diag_stage = [&](int o0, int o1) {
     if constexpr (VARIANT != kLayers) {
       const int t = opaque(tid);
+      const ThreadPart tp = thread_part(t);
 #pragma unroll 1
       for (int k = 0; k < kPairs; ++k) {
-        const PairAt p = pair_at(t, k);
+        const PairAt p = pair_at(tp, t, k);
         if (!p.valid) continue;
         T x0 = tile[p.lds], x1 = tile[p.lds ^ 1];
-        const uint64_t g0 = p.g, g1 = p.g | 1;
 #pragma unroll 1
-        for (int o = o0; o < o1; ++o) {
-          const ua_fused_op op = load_op(ops + o);
-          switch (op.kind) {
-            case UA_FOP_PHASE: {  // data: 2^n real angles per state, data_bstride in angles
-              const R* th = static_cast<const R*>(op.data) + p.b * op.data_bstride;
-              const Pair<R> t = *reinterpret_cast<const Pair<R>*>(th + p.g);
-              x0 = cmul(expmi(t.v[0]), x0);
-              x1 = cmul(expmi(t.v[1]), x1);
-              break;
-            }
-            case UA_FOP_CZ: {
-              if ((g0 >> op.a) & (g0 >> op.b) & 1) x0 = cneg(x0);
-              if ((g1 >> op.a) & (g1 >> op.b) & 1) x1 = cneg(x1);
-              break;
-            }
-            case UA_FOP_DIAG1: {
-              const R* d = static_cast<const R*>(op.data) + 2 * p.b * op.data_bstride;
-              const T d0 = ldc<T>(d, 0), d1 = ldc<T>(d, 1);
-              x0 = cmul(((g0 >> op.a) & 1) ? d1 : d0, x0);
-              x1 = cmul(((g1 >> op.a) & 1) ? d1 : d0, x1);
-              break;
-            }
-            case UA_FOP_CZ_LAYER: {
-              const uint64_t mask =
-                  static_cast<const uint64_t*>(op.data)[p.b * op.data_bstride];
-              if (__popcll(g0 & (g0 >> 1) & mask) & 1) x0 = cneg(x0);
-              if (__popcll(g1 & (g1 >> 1) & mask) & 1) x1 = cneg(x1);
-              break;
-            }
-            default:
-              break;
-          }
-        }
+        for (int o = o0; o < o1; ++o) diag_pair(load_op(ops + o), p, x0, x1);
         tile[p.lds] = x0;
         tile[p.lds ^ 1] = x1;
       }
     }
   };
 
+  // Diagonal stage of PHASE_FACTORS ops only, on the registers of the
+  // global load or store (a.reg_head / a.reg_tail are set only then): the
+  // factor pairs of an op are all in flight at once.
+  auto diag_regs = [&](T (&c)[kCube], int t, int o0, int o1) {
+    if constexpr (VARIANT != kLayers) {
+      const ThreadPart tp = thread_part(t);
+#pragma unroll 1
+      for (int o = o0; o < o1; ++o) {
+        const ua_fused_op op = load_op(ops + o);
+        const T* f = static_cast<const T*>(op.data);
+#pragma unroll
+        for (int k = 0; k < kPairs; ++k) {
+          const PairAt p = pair_at(tp, t, k);
+          if (p.valid) {
+            const Pair<T> v =
+                *reinterpret_cast<const Pair<T>*>(f + p.b * op.data_bstride + p.g);
+            c[2 * k] = cmul(v.v[0], c[2 * k]);
+            c[2 * k + 1] = cmul(v.v[1], c[2 * k + 1]);
+          }
+        }
+      }
+    }
+  };
+
   // ---- coalesced load of this workgroup's tiles (pairs of amplitudes)
   T c[kCube];
+  {
+    const ThreadPart tp = thread_part(tid);
 #pragma unroll
-  for (int k = 0; k < kPairs; ++k) {
-    const PairAt p = pair_at(tid, k);
-    if (p.valid) {
-      const Pair<T> v = *reinterpret_cast<const Pair<T>*>(state + p.b * N + p.g);
-      c[2 * k] = v.v[0];
-      c[2 * k + 1] = v.v[1];
-    } else {
-      c[2 * k].x = c[2 * k].y = 0;
-      c[2 * k + 1] = c[2 * k];
+    for (int k = 0; k < kPairs; ++k) {
+      const PairAt p = pair_at(tp, tid, k);
+      if (p.valid) {
+        const Pair<T> v = *reinterpret_cast<const Pair<T>*>(state + p.b * N + p.g);
+        c[2 * k] = v.v[0];
+        c[2 * k + 1] = v.v[1];
+      } else {
+        c[2 * k].x = c[2 * k].y = 0;
+        c[2 * k + 1] = c[2 * k];
+      }
     }
   }
+  auto store = [&](int t) {
+    const ThreadPart tp = thread_part(t);
 #pragma unroll
-  for (int k = 0; k < kPairs; ++k) {
-    const PairAt p = pair_at(tid, k);
-    tile[p.lds] = c[2 * k];
-    tile[p.lds ^ 1] = c[2 * k + 1];
+    for (int k = 0; k < kPairs; ++k) {
+      const PairAt p = pair_at(tp, t, k);
+      if (p.valid) {
+        Pair<T> v;
+        v.v[0] = c[2 * k];
+        v.v[1] = c[2 * k + 1];
+        *reinterpret_cast<Pair<T>*>(state + p.b * N + p.g) = v;
+      }
+    }
+  };
+  // a leading / trailing diagonal stage runs on these registers
+  const int st_begin = a.reg_head ? 1 : 0;
+  const int st_end = a.num_stages - ((a.reg_tail && a.num_stages > st_begin) ? 1 : 0);
+  const bool tail = st_end < a.num_stages;
+  if (a.reg_head) diag_regs(c, tid, a.stage_ops[0], a.stage_ops[1]);
+  if (st_begin >= st_end) {  // nothing goes through LDS
+    if (tail) diag_regs(c, tid, a.stage_ops[a.num_stages - 1], a.stage_ops[a.num_stages]);
+    store(tid);
+    return;
+  }
+  {
+    const ThreadPart tp = thread_part(tid);
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+      const PairAt p = pair_at(tp, tid, k);
+      tile[p.lds] = c[2 * k];
+      tile[p.lds ^ 1] = c[2 * k + 1];
+    }
   }
 
   // ---- this thread's place in its tile for cube stages
@@ -375,7 +455,7 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd<T>()))) void fused_pass_kernel
   if (a.needs_index && active) tile_base = MULTI ? tile_origin(tg_c) : origin1;
   char* const tile_bytes = reinterpret_cast<char*>(tile);
 
-  for (int st = 0; st < a.num_stages; ++st) {
+  for (int st = st_begin; st < st_end; ++st) {
     __syncthreads();
     const int s0 = a.stage_bits[st][0];
     if (s0 < 0) {  // diagonal stage: the thread's own pairs, in tile order
@@ -507,21 +587,22 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd<T>()))) void fused_pass_kernel
   // ---- coalesced store (same element order as the load)
   __syncthreads();
   const int t_end = opaque(tid);
+  {
+    const ThreadPart tp = thread_part(t_end);
 #pragma unroll
-  for (int k = 0; k < kPairs; ++k) {
-    const PairAt p = pair_at(t_end, k);
-    if (p.valid) {
-      Pair<T> v;
-      v.v[0] = tile[p.lds];
-      v.v[1] = tile[p.lds ^ 1];
-      *reinterpret_cast<Pair<T>*>(state + p.b * N + p.g) = v;
+    for (int k = 0; k < kPairs; ++k) {
+      const PairAt p = pair_at(tp, t_end, k);
+      c[2 * k] = tile[p.lds];
+      c[2 * k + 1] = tile[p.lds ^ 1];
     }
   }
+  if (tail) diag_regs(c, t_end, a.stage_ops[a.num_stages - 1], a.stage_ops[a.num_stages]);
+  store(t_end);
 }
 
 bool is_diag_kind(int kind) {
   return kind == UA_FOP_CZ || kind == UA_FOP_DIAG1 || kind == UA_FOP_PHASE ||
-         kind == UA_FOP_CZ_LAYER;
+         kind == UA_FOP_CZ_LAYER || kind == UA_FOP_PHASE_FACTORS;
 }
 
 }  // namespace
@@ -558,6 +639,7 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
                  "local bits must be ascending and start 0,1,2,3");
   }
   bool heavy = false, only_layers = true, needs_index = false;
+  bool stage_not_factors[UA_FUSED_MAX_STAGES] = {};
   for (int st = 0; st < a.num_stages; ++st) {
     UA_CHECK_ARG(pass->stage_ops[st + 1] >= pass->stage_ops[st], "stage %d: bad op range", st);
     const bool diag = pass->stage_bits[st][0] < 0;
@@ -580,12 +662,15 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
     a.stage_ops[st] = pass->stage_ops[st];
     for (int o = pass->stage_ops[st]; o < pass->stage_ops[st + 1]; ++o) {
       const ua_fused_op& op = host_ops[o];
-      UA_CHECK_ARG(op.kind >= UA_FOP_DENSE1 && op.kind <= UA_FOP_LAYER1Q, "op %d: bad kind", o);
+      UA_CHECK_ARG(op.kind >= UA_FOP_DENSE1 && op.kind <= UA_FOP_PHASE_FACTORS,
+                   "op %d: bad kind", o);
       if (diag) {
-        UA_CHECK_ARG(is_diag_kind(op.kind), "op %d: a diagonal stage takes CZ, DIAG1, PHASE "
-                     "and CZ_LAYER ops", o);
+        UA_CHECK_ARG(is_diag_kind(op.kind), "op %d: a diagonal stage takes CZ, DIAG1, PHASE, "
+                     "PHASE_FACTORS and CZ_LAYER ops", o);
+        if (op.kind != UA_FOP_PHASE_FACTORS) stage_not_factors[st] = true;
       } else {
-        UA_CHECK_ARG(op.kind != UA_FOP_PHASE, "op %d: PHASE ops run in diagonal stages", o);
+        UA_CHECK_ARG(op.kind != UA_FOP_PHASE && op.kind != UA_FOP_PHASE_FACTORS,
+                     "op %d: PHASE ops run in diagonal stages", o);
         needs_index = needs_index || (op.kind != UA_FOP_DENSE1 && op.kind != UA_FOP_DENSE2 &&
                                       op.kind != UA_FOP_LAYER1Q);
       }
@@ -625,10 +710,17 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
         UA_CHECK_ARG(reinterpret_cast<uintptr_t>(op.data) % (2 * rsize) == 0 &&
                          op.data_bstride % 2 == 0,
                      "op %d: phase angles must be aligned to two angles", o);
+      if (op.kind == UA_FOP_PHASE_FACTORS)
+        UA_CHECK_ARG(reinterpret_cast<uintptr_t>(op.data) % (4 * rsize) == 0 &&
+                         op.data_bstride % 2 == 0,
+                     "op %d: phase factors must be aligned to two elements", o);
     }
   }
   a.stage_ops[a.num_stages] = pass->stage_ops[a.num_stages];
   a.needs_index = needs_index ? 1 : 0;
+  const int ns = a.num_stages;
+  a.reg_head = ns > 0 && a.stage_bits[0][0] < 0 && !stage_not_factors[0];
+  a.reg_tail = ns > 0 && a.stage_bits[ns - 1][0] < 0 && !stage_not_factors[ns - 1];
   const int64_t tiles_per_block = int64_t(kTile) >> m;
   const int64_t blocks = (a.total_tiles + tiles_per_block - 1) / tiles_per_block;
   UA_CHECK_ARG(blocks <= 0x7fffffffLL, "too many tiles");

@@ -31,7 +31,7 @@ CUBE_BITS = 4
 MAX_STAGES = _lib.UA_FUSED_MAX_STAGES
 
 (FOP_DENSE1, FOP_DENSE2, FOP_CNOT, FOP_CZ, FOP_DIAG1, FOP_PHASE, FOP_CZ_LAYER,
- FOP_LAYER1Q) = range(8)
+ FOP_LAYER1Q, FOP_PHASE_FACTORS) = range(9)
 
 
 FusedOp = _lib.FusedOp
@@ -354,7 +354,7 @@ class FusedProgram:
                             o.kind, o.a = FOP_DIAG1, it.bits[0]
                             needs_index = 1
                         elif it.kind == "phase":
-                            o.kind = FOP_PHASE
+                            o.kind = FOP_PHASE_FACTORS
                             needs_index = 1
                         elif it.kind == "cz_layer":
                             o.kind = FOP_CZ_LAYER
@@ -382,6 +382,13 @@ class FusedProgram:
                 t = it.data.to(self.device, want).contiguous()
             else:
                 t = torch.as_tensor(np.ascontiguousarray(it.data)).to(self.device, want)
+            if it.kind == "phase":
+                # exp(-1j * angles), formed once by the drop-in phase kernel
+                # (the same factors apply_phase multiplies by); the pass
+                # then multiplies by them
+                from .simulation import apply_phase
+                t = apply_phase(t, torch.ones(2 ** self.n, dtype=dtype, device=self.device))
+                t = t.contiguous()
             per = {"phase": 2 ** self.n, "diag1": 2, "dense1": 4, "dense2": 16,
                    "cz_layer": 1}[it.kind]
             o.data_bstride = per if it.per_state else 0

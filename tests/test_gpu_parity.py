@@ -226,20 +226,30 @@ def test_reductions(device, dt):
 
 # ----------------------------------------------------- CZ / CNOT in place --
 
-@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
-def test_cz_cnot_bit_exact(device, dt):
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128, torch.float32,
+                                torch.float64])
+@pytest.mark.parametrize("misaligned", [False, True])
+def test_cz_cnot_bit_exact(device, dt, misaligned):
+    """In-place CZ/CNOT equal the dense 4x4 application bit for bit, for every
+    role of index bit 0 (qubit n-1: both qubits' pair forms, control or
+    target of CNOT) and on a state that starts one element off a pair
+    boundary (the element-wise fallback)."""
     from unitair_amd import _lib, gates
     lib = _lib.load()
     n = 9
-    st = torch.randn(3, 2 ** n, dtype=dt, device=device)
-    for a, b in [(0, 8), (8, 0), (3, 7), (7, 3), (1, 2), (8, 7)]:
+    base = torch.randn(3 * 2 ** n + 1, dtype=dt, device=device)
+    st = (base[1:] if misaligned else base[:-1]).view(3, 2 ** n)
+    for a, b in [(0, 8), (8, 0), (3, 7), (7, 3), (1, 2), (8, 7), (7, 8), (8, 1)]:
         for kind, mat, fn in (("cz", gates.cz(device=device, dtype=dt), lib.ua_apply_cz),
                               ("cnot", gates.cnot(device=device, dtype=dt), lib.ua_apply_cnot)):
-            dense = sim.apply_operator(mat, (a, b), st)
-            x = st.clone()
+            dense = sim.apply_operator(mat, (a, b), st.contiguous())
+            keep = base.clone()
+            x = (keep[1:] if misaligned else keep[:-1]).view(3, 2 ** n)
             rc = fn(_lib.ptr(x), _lib.DTYPE_CODES[dt], n, a, b, 3, _lib.stream_handle(device))
             _lib.check(rc, kind)
             assert torch.equal(x, dense), (kind, a, b)
+            edge = 0 if misaligned else -1       # the element outside the state is untouched
+            assert torch.equal(keep[edge], base[edge])
 
 
 # ----------------------------------------------------------------- circuits --
