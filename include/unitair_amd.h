@@ -228,6 +228,21 @@ int ua_norm_squared(void* out, int out_dtype, const void* state, int dtype,
                     int64_t batch, int64_t length, int64_t row_stride,
                     void* workspace, ua_stream_t stream);
 
+/*
+ * ua_norm_squared_noalloc -- ua_norm_squared with no workspace argument and no
+ * allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * The float64 partials live in a static device array of 2^16 (re, im) pairs
+ * (1 MiB, held by every GPU the library runs on until the process ends), so
+ * batch * ceil(length / 4096) must be at most 65536 (2^28 elements per call);
+ * larger calls fail with UA_ERR_INVALID_ARGUMENT.  The array's address on the
+ * current device is looked up on the first call on that device.  Calls on one
+ * GPU share the array: calls that overlap on one GPU (different streams) race.
+ */
+int ua_norm_squared_noalloc(void* out, int out_dtype, const void* state, int dtype,
+                            int64_t batch, int64_t length, int64_t row_stride,
+                            ua_stream_t stream);
+
 /* inner_product (innerprod.py:62-65): sum_x conj(a_x) b_x.  out_dtype
  * complex; a and b share `dtype`. */
 int ua_inner_product(void* out, int out_dtype, const void* a, int64_t a_row_stride,

@@ -6,6 +6,8 @@
 // fixed order and converts to the requested output dtype.
 #include "ua_common.h"
 
+#include <mutex>
+
 namespace ua {
 namespace {
 
@@ -178,6 +180,70 @@ int norm_squared_impl(void* out, int odt, const void* st, int dt, int64_t batch,
 }
 
 }  // namespace ua
+
+// ---------------------------------------------------------------------------
+// ua_norm_squared_noalloc: ua_norm_squared with its float64 partials in a
+// static device array instead of a caller workspace, for callers that cannot
+// allocate (code under graph capture, tight serving loops).  Every GPU holds
+// its own copy of the array for the life of the process; calls on one GPU
+// share it, so calls that overlap on one GPU race.  Nothing in it needs to
+// start at zero: pass 1 writes every partial that pass 2 reads.
+namespace {
+constexpr int64_t kScratchPartials = int64_t(1) << 16;  // 1 MiB of (re, im) float64 pairs
+constexpr int kScratchDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double2 scratch[kScratchPartials];
+
+namespace {
+std::mutex g_scratch_mu;
+double2* g_scratch_ptr[kScratchDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current (the name itself is a host-side stand-in).
+int scratch_of_current_device(double2** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    ua::set_error("hipGetDevice failed: %s", hipGetErrorString(e));
+    return UA_ERR_HIP;
+  }
+  UA_CHECK_ARG(dev >= 0 && dev < kScratchDevices, "device %d beyond the %d supported", dev,
+               kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      ua::set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return UA_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double2*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return UA_OK;
+}
+}  // namespace
+
+extern "C" int ua_norm_squared_noalloc(void* out, int out_dtype, const void* state, int dtype,
+                                       int64_t batch, int64_t length, int64_t row_stride,
+                                       ua_stream_t stream) {
+  using namespace ua;
+  UA_CHECK_ARG(elem_size(out_dtype) != 0, "bad output dtype %d", out_dtype);
+  UA_CHECK_ARG(elem_size(dtype) != 0, "bad dtype %d", dtype);
+  UA_CHECK_ARG(batch >= 0 && length >= 0, "negative size");
+  if (batch == 0) return UA_OK;
+  const int64_t chunks = chunks_for(length);
+  UA_CHECK_ARG(chunks <= kScratchPartials && batch <= kScratchPartials / chunks,
+               "batch %lld x %lld chunks of %d elements exceeds the static scratch of %lld "
+               "partials (at most 2^28 elements per call)",
+               (long long)batch, (long long)chunks, kChunk, (long long)kScratchPartials);
+  UA_CHECK_ARG(out && state, "NULL output or state pointer");
+  double2* ws = nullptr;
+  if (int rc = scratch_of_current_device(&ws)) return rc;
+  return norm_squared_impl(out, out_dtype, state, dtype, batch, length, row_stride, ws,
+                           static_cast<hipStream_t>(stream));
+}
 
 extern "C" int64_t ua_reduce_workspace_bytes(int64_t batch, int64_t length) {
   return ua::reduce_workspace_bytes(batch, length);

@@ -62,6 +62,7 @@ SIGNATURES = {
     "ua_apply_controlled_flip": (_i32, [_p, _p, _i32, _i32, _u64, _i32, _i64, _i64, _p]),
     "ua_reduce_workspace_bytes": (_i64, [_i64, _i64]),
     "ua_norm_squared": (_i32, [_p, _i32, _p, _i32, _i64, _i64, _i64, _p, _p]),
+    "ua_norm_squared_noalloc": (_i32, [_p, _i32, _p, _i32, _i64, _i64, _i64, _p]),
     "ua_inner_product": (_i32, [_p, _i32, _p, _i64, _p, _i64, _i32, _i64, _i64, _p, _p]),
     "ua_diag_expectation": (_i32, [_p, _i32, _p, _i32, _i64, _p, _i32, _i64, _i64, _i64, _p, _p]),
     "ua_abs_squared": (_i32, [_p, _p, _i32, _i64, _p]),

@@ -126,3 +126,19 @@ def test_validation_errors_of_layer_permutation_sampling_and_constructors(lib):
     # zero-sized work succeeds without touching memory
     assert lib.ua_apply_1q_layer(None, None, None, c64, 12, 0, None, 0, 4096, 0, None) == _lib.UA_OK
     assert lib.ua_sample(None, None, c64, 4, 0, 0, 0, None, None) == _lib.UA_OK
+
+
+def test_norm_squared_noalloc_rejects_sizes_beyond_its_scratch(lib):
+    """ua_norm_squared_noalloc checks its size limit (batch x ceil(length/4096)
+    partials <= 2^16) before it touches any device: sizes beyond it fail with
+    UA_ERR_INVALID_ARGUMENT and a message naming the static scratch."""
+    from unitair_amd import _lib
+    bad = _lib.UA_ERR_INVALID_ARGUMENT
+    c64 = _lib.UA_COMPLEX64
+    f64 = _lib.UA_FLOAT64
+    assert lib.ua_norm_squared_noalloc(1, f64, 1, c64, 17, 2 ** 24, 2 ** 24, None) == bad
+    assert b"static scratch" in lib.ua_last_error()
+    assert lib.ua_norm_squared_noalloc(1, f64, 1, c64, 1, 2 ** 28 + 1, 0, None) == bad
+    assert lib.ua_norm_squared_noalloc(1, 99, 1, c64, 1, 16, 16, None) == bad
+    assert lib.ua_norm_squared_noalloc(1, f64, 1, c64, -1, 16, 16, None) == bad
+    assert lib.ua_norm_squared_noalloc(None, f64, None, c64, 0, 16, 16, None) == _lib.UA_OK
