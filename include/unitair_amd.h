@@ -415,13 +415,14 @@ int ua_operator_gradient(void* grad_op, const void* grad_out, const void* state,
  * ua_expect_sum_z   out[b] = sum_x |psi_b(x)|^2 (n - 2 popcount x)   (float64)
  * ua_apply_sum_z    out = H psi (the co-state at the end of the circuit)
  * ua_pauli_gradients out[b*n + q] = Im <lambda_b| P_q |psi_b> for every qubit
- *                   q, P = Y or Z: the derivative of the loss w.r.t. the angle
- *                   t of a rotation exp(-i t/2 P_q) sitting where (lambda, psi)
- *                   were taken.  Y needs num_qubits <= 16.
+ *                   q, P = X, Y or Z: the derivative of <psi|H|psi> w.r.t. the
+ *                   angle t of a rotation exp(-i t/2 P_q) sitting where (lambda,
+ *                   psi) were taken (twice that for unitair's exp_x/exp_y/exp_z(t)
+ *                   = exp(-i t P), gates.py:7-91).
  * Workspace: ua_pauli_gradients_workspace(num_qubits, batch) bytes (both
  * ua_expect_sum_z and ua_pauli_gradients).  complex64/complex128 states.
  */
-enum { UA_PAULI_Y = 1, UA_PAULI_Z = 2 };
+enum { UA_PAULI_X = 0, UA_PAULI_Y = 1, UA_PAULI_Z = 2 };
 int64_t ua_pauli_gradients_workspace(int num_qubits, int64_t batch);
 int ua_expect_sum_z(double* out, const void* psi, int dtype, int num_qubits, int64_t batch,
                     void* workspace, ua_stream_t stream);
@@ -430,6 +431,15 @@ int ua_apply_sum_z(void* out, const void* psi, int dtype, int num_qubits, int64_
 int ua_pauli_gradients(double* out, const void* lambda, const void* psi, int dtype,
                        int num_qubits, int64_t batch, int pauli, void* workspace,
                        ua_stream_t stream);
+
+/* ua_apply_diagonal  out = d * psi elementwise for a real diagonal d (float32
+ *                    or float64; diag_batch_stride 0 = one row for every state,
+ *                    else elements between rows): the co-state H psi of a
+ *                    diagonal observable (the loss diag_expectation_value(d,
+ *                    psi), innerprod.py:49-59). */
+int ua_apply_diagonal(void* out, const void* psi, int dtype, const void* diag, int diag_dtype,
+                      int64_t diag_batch_stride, int num_qubits, int64_t batch,
+                      ua_stream_t stream);
 
 #ifdef __cplusplus
 }

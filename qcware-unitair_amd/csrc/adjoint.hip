@@ -54,11 +54,22 @@ __device__ double block_sum(double v, double* scratch) {
   return r;
 }
 
-// Y gradients for index bits 0..min(n,12)-1.  One workgroup per (state,
+// Im <lambda| P_b |psi> summed over one amplitude pair (x0 with bit b clear,
+// x1 = x0 | 2^b), for P = X or Y:
+//   X: (P psi)_x = psi_{x^b}             -> Im(l0* p1) + Im(l1* p0)
+//   Y: (P psi)_{x0} = -i psi_{x1},
+//      (P psi)_{x1} =  i psi_{x0}        -> Re(l1* p0) - Re(l0* p1)
+template <int P>
+__device__ __forceinline__ double pair_term(double2 l0, double2 l1, double2 p0, double2 p1) {
+  if constexpr (P == UA_PAULI_X) return im_cj(l0, p1) + im_cj(l1, p0);
+  else return re_cj(l1, p0) - re_cj(l0, p1);
+}
+
+// X / Y gradients for index bits 0..min(n,12)-1.  One workgroup per (state,
 // tile of 2^m contiguous amplitudes); lambda and psi tiles sit in LDS.
 // partial[(b * tiles + tile) * n + bit] = sum over the tile's pairs.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void grad_y_low_kernel(
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void grad_low_kernel(
     double* __restrict__ partial, const T* __restrict__ lam, const T* __restrict__ psi,
     int n, int m, int64_t tiles_per_state) {
   __shared__ T sl[kLowTile];
@@ -70,10 +81,10 @@ __global__ __launch_bounds__(kBlock) void grad_y_low_kernel(
   const int64_t N = int64_t(1) << n;
   const int size = 1 << m;
   const T* L = lam + b * N + t * size;
-  const T* P = psi + b * N + t * size;
+  const T* Ps = psi + b * N + t * size;
   for (int i = threadIdx.x; i < size; i += kBlock) {
     sl[i] = L[i];
-    sp[i] = P[i];
+    sp[i] = Ps[i];
   }
   __syncthreads();
   for (int bit = 0; bit < m; ++bit) {
@@ -82,43 +93,58 @@ __global__ __launch_bounds__(kBlock) void grad_y_low_kernel(
     for (int pidx = threadIdx.x; pidx < half; pidx += kBlock) {
       const int x0 = ((pidx >> bit) << (bit + 1)) | (pidx & ((1 << bit) - 1));
       const int x1 = x0 | (1 << bit);
-      const double2 l0 = D2<T>::get(sl[x0]), l1 = D2<T>::get(sl[x1]);
-      const double2 p0 = D2<T>::get(sp[x0]), p1 = D2<T>::get(sp[x1]);
-      acc += re_cj(l1, p0) - re_cj(l0, p1);
+      acc += pair_term<P>(D2<T>::get(sl[x0]), D2<T>::get(sl[x1]), D2<T>::get(sp[x0]),
+                          D2<T>::get(sp[x1]));
     }
     const double s = block_sum(acc, scratch);
     if (threadIdx.x == 0) partial[(b * tiles_per_state + t) * n + bit] = s;
   }
 }
 
-// Y gradients for index bits >= 12 (n <= 16): each thread owns one low index
-// and the 2^(n-12) amplitudes that share it, read with a 2^12 stride.
-// partial[(b * 16 + blockIdx.y) * n + bit] for bits 12..n-1.
-template <typename T, int H>
-__global__ __launch_bounds__(kBlock) void grad_y_high_kernel(
-    double* __restrict__ partial, const T* __restrict__ lam, const T* __restrict__ psi, int n) {
+// X / Y gradients for a window of H index bits w0..w0+H-1 (w0 >= 12).  Each
+// thread owns one low index (bits 0..11: 256 consecutive per workgroup, so
+// every load is coalesced) and, for each value r of the other high bits in
+// its block's range, the 2^H amplitudes over the window.
+// partial[((b * 16 + blockIdx.y) * gridDim.z + blockIdx.z) * n + bit].
+template <typename T, int P, int H>
+__global__ __launch_bounds__(kBlock) void grad_high_kernel(
+    double* __restrict__ partial, const T* __restrict__ lam, const T* __restrict__ psi, int n,
+    int w0, int64_t rest_per_block) {
   __shared__ double scratch[kBlock / 64];
   constexpr int HN = 1 << H;
   const int64_t b = blockIdx.x;
   const int low = blockIdx.y * kBlock + threadIdx.x;  // < 4096
   const int64_t N = int64_t(1) << n;
-  double2 l[HN], p[HN];
+  const int below = w0 - kLowBits;                    // other high bits under the window
+  double acc[H];
 #pragma unroll
-  for (int h = 0; h < HN; ++h) {
-    l[h] = D2<T>::get(lam[b * N + (int64_t(h) << kLowBits) + low]);
-    p[h] = D2<T>::get(psi[b * N + (int64_t(h) << kLowBits) + low]);
-  }
-#pragma unroll
-  for (int hb = 0; hb < H; ++hb) {
-    double acc = 0.0;
+  for (int hb = 0; hb < H; ++hb) acc[hb] = 0.0;
+  const int64_t r0 = int64_t(blockIdx.z) * rest_per_block;
+  for (int64_t r = r0; r < r0 + rest_per_block; ++r) {
+    const uint64_t rest = ((uint64_t(r) & ((uint64_t(1) << below) - 1)) << kLowBits) |
+                          ((uint64_t(r) >> below) << (w0 + H));
+    const int64_t base = b * N + int64_t(rest) + low;
+    double2 l[HN], p[HN];
 #pragma unroll
     for (int h = 0; h < HN; ++h) {
-      if (h & (1 << hb)) continue;
-      const int h1 = h | (1 << hb);
-      acc += re_cj(l[h1], p[h]) - re_cj(l[h], p[h1]);
+      l[h] = D2<T>::get(lam[base + (int64_t(h) << w0)]);
+      p[h] = D2<T>::get(psi[base + (int64_t(h) << w0)]);
     }
-    const double s = block_sum(acc, scratch);
-    if (threadIdx.x == 0) partial[(b * gridDim.y + blockIdx.y) * n + kLowBits + hb] = s;
+#pragma unroll
+    for (int hb = 0; hb < H; ++hb) {
+#pragma unroll
+      for (int h = 0; h < HN; ++h) {
+        if (h & (1 << hb)) continue;
+        const int h1 = h | (1 << hb);
+        acc[hb] += pair_term<P>(l[h], l[h1], p[h], p[h1]);
+      }
+    }
+  }
+  const int64_t part = (b * gridDim.y + blockIdx.y) * gridDim.z + blockIdx.z;
+#pragma unroll
+  for (int hb = 0; hb < H; ++hb) {
+    const double s = block_sum(acc[hb], scratch);
+    if (threadIdx.x == 0) partial[part * n + w0 + hb] = s;
   }
 }
 
@@ -209,7 +235,21 @@ __global__ __launch_bounds__(kBlock) void sum_z_apply_kernel(T* __restrict__ out
 }
 
 int64_t y_low_parts(int n) { return n > kLowBits ? (int64_t(1) << (n - kLowBits)) : 1; }
-constexpr int64_t kHighParts = kLowTile / kBlock;  // 16
+constexpr int64_t kHighParts = kLowTile / kBlock;  // 16 low chunks per state
+constexpr int kHighWindow = 4;                      // index bits per high-bit launch
+// Splits of the other high bits of a window launch across blockIdx.z: enough
+// workgroups to fill the GPU (about 2048 per launch) with at least one value
+// of the other bits each.
+int64_t high_z(int n, int64_t batch) {
+  const int rest_bits = n - kLowBits - kHighWindow;
+  if (rest_bits <= 0) return 1;
+  int64_t z = (2048 + batch * kHighParts - 1) / (batch * kHighParts);
+  const int64_t rest = int64_t(1) << rest_bits;
+  if (z > rest) z = rest;
+  int64_t p2 = 1;
+  while (p2 * 2 <= z) p2 *= 2;   // a power of two divides the rest count
+  return p2;
+}
 int64_t z_parts(int n) {
   return ((int64_t(1) << n) + kZChunkConst - 1) / kZChunkConst;
 }
@@ -223,7 +263,7 @@ extern "C" int64_t ua_pauli_gradients_workspace(int num_qubits, int64_t batch) {
   const int n = num_qubits;
   if (n < 1 || n > kMaxN || batch < 0) return -1;
   int64_t a = y_low_parts(n) * n;
-  const int64_t h = kHighParts * n;
+  const int64_t h = kHighParts * high_z(n, batch < 1 ? 1 : batch) * n;
   const int64_t z = z_parts(n) * (n + 1);
   if (h > a) a = h;
   if (z > a) a = z;
@@ -237,7 +277,8 @@ extern "C" int ua_pauli_gradients(double* out, const void* lambda, const void* p
   hipStream_t s = static_cast<hipStream_t>(stream);
   UA_CHECK_ARG(dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128, "complex states only");
   UA_CHECK_ARG(n >= 1 && n <= kMaxN, "num_qubits=%d out of range", n);
-  UA_CHECK_ARG(pauli == UA_PAULI_Y || pauli == UA_PAULI_Z, "pauli must be Y or Z");
+  UA_CHECK_ARG(pauli == UA_PAULI_X || pauli == UA_PAULI_Y || pauli == UA_PAULI_Z,
+               "pauli must be X, Y or Z");
   UA_CHECK_ARG(batch >= 0, "negative batch");
   if (batch == 0) return UA_OK;
   UA_CHECK_ARG(out && lambda && psi && workspace, "NULL pointer");
@@ -260,18 +301,22 @@ extern "C" int ua_pauli_gradients(double* out, const void* lambda, const void* p
                                                             n, 0, 0);
     return check_launch("sum_parts_kernel");
   }
-  // Y: index bit i belongs to qubit n-1-i; partials are kept per index bit
-  // and mapped to qubits when summed.
+  // X / Y: index bit i belongs to qubit n-1-i; partials are kept per index
+  // bit and mapped to qubits when summed.
   const int m = n < kLowBits ? n : kLowBits;
   const int64_t tiles = y_low_parts(n);
   UA_CHECK_ARG(batch * tiles <= 0x7fffffffLL, "grid too large");
-  if (c64)
-    grad_y_low_kernel<float2><<<unsigned(batch * tiles), kBlock, 0, s>>>(
-        part, (const float2*)lambda, (const float2*)psi, n, m, tiles);
-  else
-    grad_y_low_kernel<double2><<<unsigned(batch * tiles), kBlock, 0, s>>>(
-        part, (const double2*)lambda, (const double2*)psi, n, m, tiles);
-  int rc = check_launch("grad_y_low_kernel");
+  const unsigned glow = unsigned(batch * tiles);
+#define UA_LOW(TT, PP)                                                                  \
+  grad_low_kernel<TT, PP><<<glow, kBlock, 0, s>>>(part, (const TT*)lambda, (const TT*)psi, n, \
+                                                  m, tiles)
+  if (pauli == UA_PAULI_X) {
+    if (c64) UA_LOW(float2, UA_PAULI_X); else UA_LOW(double2, UA_PAULI_X);
+  } else {
+    if (c64) UA_LOW(float2, UA_PAULI_Y); else UA_LOW(double2, UA_PAULI_Y);
+  }
+#undef UA_LOW
+  int rc = check_launch("grad_low_kernel");
   if (rc) return rc;
   // bits 0..m-1 -> qubits n-1 .. n-m   (sum_parts writes q-th bit at out index)
   for (int bit = 0; bit < m; ++bit) {
@@ -280,32 +325,92 @@ extern "C" int ua_pauli_gradients(double* out, const void* lambda, const void* p
     rc = check_launch("sum_parts_kernel");
     if (rc) return rc;
   }
-  if (n > kLowBits) {
-    const int H = n - kLowBits;
-    UA_CHECK_ARG(H <= 4, "Y gradients support up to 16 qubits");
-    UA_CHECK_ARG(batch <= 0x7fffffffLL, "grid too large");
-    const dim3 grid = dim3(static_cast<unsigned>(batch), static_cast<unsigned>(kHighParts), 1u);
-#define UA_HIGH(HH)                                                                       \
-  if (H == HH) {                                                                         \
-    if (c64)                                                                             \
-      grad_y_high_kernel<float2, HH><<<grid, kBlock, 0, s>>>(part, (const float2*)lambda, \
-                                                             (const float2*)psi, n);     \
-    else                                                                                 \
-      grad_y_high_kernel<double2, HH><<<grid, kBlock, 0, s>>>(                           \
-          part, (const double2*)lambda, (const double2*)psi, n);                         \
+  // high bits, kHighWindow at a time; each launch reads both states once
+  UA_CHECK_ARG(batch <= 0x7fffffffLL, "grid too large");
+  const int64_t z = high_z(n, batch);
+  for (int w0 = kLowBits; w0 < n; w0 += kHighWindow) {
+    const int H = (n - w0) < kHighWindow ? (n - w0) : kHighWindow;
+    const int rest_bits = n - kLowBits - H;
+    const int64_t zz = rest_bits > 0 ? z : 1;
+    const int64_t per_block = (int64_t(1) << (rest_bits > 0 ? rest_bits : 0)) / zz;
+    const dim3 grid = dim3(unsigned(batch), unsigned(kHighParts), unsigned(zz));
+#define UA_HIGH(HH)                                                                             \
+  if (H == HH) {                                                                               \
+    if (pauli == UA_PAULI_X) {                                                                 \
+      if (c64)                                                                                 \
+        grad_high_kernel<float2, UA_PAULI_X, HH><<<grid, kBlock, 0, s>>>(                      \
+            part, (const float2*)lambda, (const float2*)psi, n, w0, per_block);                \
+      else                                                                                     \
+        grad_high_kernel<double2, UA_PAULI_X, HH><<<grid, kBlock, 0, s>>>(                     \
+            part, (const double2*)lambda, (const double2*)psi, n, w0, per_block);              \
+    } else {                                                                                   \
+      if (c64)                                                                                 \
+        grad_high_kernel<float2, UA_PAULI_Y, HH><<<grid, kBlock, 0, s>>>(                      \
+            part, (const float2*)lambda, (const float2*)psi, n, w0, per_block);                \
+      else                                                                                     \
+        grad_high_kernel<double2, UA_PAULI_Y, HH><<<grid, kBlock, 0, s>>>(                     \
+            part, (const double2*)lambda, (const double2*)psi, n, w0, per_block);              \
+    }                                                                                          \
   }
     UA_HIGH(1) UA_HIGH(2) UA_HIGH(3) UA_HIGH(4)
 #undef UA_HIGH
-    rc = check_launch("grad_y_high_kernel");
+    rc = check_launch("grad_high_kernel");
     if (rc) return rc;
-    for (int bit = kLowBits; bit < n; ++bit) {
-      sum_parts_kernel<<<grid_for(batch), kBlock, 0, s>>>(out, part, batch, kHighParts, n, bit, 1,
-                                                          n, n - 1 - bit, 0);
+    for (int bit = w0; bit < w0 + H; ++bit) {
+      sum_parts_kernel<<<grid_for(batch), kBlock, 0, s>>>(out, part, batch, kHighParts * zz, n,
+                                                          bit, 1, n, n - 1 - bit, 0);
       rc = check_launch("sum_parts_kernel");
       if (rc) return rc;
     }
   }
   return UA_OK;
+}
+
+namespace ua {
+namespace {
+// out_x = d_x psi_x with a real diagonal d (one row, or one row per state)
+template <typename T, typename D>
+__global__ __launch_bounds__(kBlock) void diag_scale_kernel(T* __restrict__ out,
+                                                            const T* __restrict__ psi,
+                                                            const D* __restrict__ diag, int n,
+                                                            int64_t total, int64_t dstride) {
+  const int64_t stride = int64_t(gridDim.x) * kBlock;
+  const uint64_t mask = (uint64_t(1) << n) - 1;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < total; i += stride) {
+    const auto w = decltype(psi[0].x)(diag[(i >> n) * dstride + int64_t(uint64_t(i) & mask)]);
+    T v = psi[i];
+    v.x *= w;
+    v.y *= w;
+    out[i] = v;
+  }
+}
+}  // namespace
+}  // namespace ua
+
+extern "C" int ua_apply_diagonal(void* out, const void* psi, int dtype, const void* diag,
+                                 int diag_dtype, int64_t diag_batch_stride, int num_qubits,
+                                 int64_t batch, ua_stream_t stream) {
+  const int n = num_qubits;
+  UA_CHECK_ARG(dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128, "complex states only");
+  UA_CHECK_ARG(diag_dtype == UA_FLOAT32 || diag_dtype == UA_FLOAT64, "real diagonal only");
+  UA_CHECK_ARG(n >= 0 && n <= 62 && batch >= 0 && diag_batch_stride >= 0, "bad sizes");
+  if (batch == 0) return UA_OK;
+  UA_CHECK_ARG(out && psi && diag, "NULL pointer");
+  const int64_t total = batch << n;
+  int64_t blocks = (total + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const unsigned g = unsigned(blocks);
+#define UA_DS(T, D)                                                                     \
+  diag_scale_kernel<T, D><<<g, kBlock, 0, s>>>((T*)out, (const T*)psi, (const D*)diag, n, \
+                                               total, diag_batch_stride)
+  if (dtype == UA_COMPLEX64) {
+    if (diag_dtype == UA_FLOAT32) UA_DS(float2, float); else UA_DS(float2, double);
+  } else {
+    if (diag_dtype == UA_FLOAT32) UA_DS(double2, float); else UA_DS(double2, double);
+  }
+#undef UA_DS
+  return check_launch("diag_scale_kernel");
 }
 
 extern "C" int ua_apply_sum_z(void* out, const void* psi, int dtype, int num_qubits,

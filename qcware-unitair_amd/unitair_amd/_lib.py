@@ -73,6 +73,7 @@ SIGNATURES = {
     "ua_expect_sum_z": (_i32, [_p, _p, _i32, _i32, _i64, _p, _p]),
     "ua_apply_sum_z": (_i32, [_p, _p, _i32, _i32, _i64, _p]),
     "ua_pauli_gradients": (_i32, [_p, _p, _p, _i32, _i32, _i64, _i32, _p, _p]),
+    "ua_apply_diagonal": (_i32, [_p, _p, _i32, _p, _i32, _i64, _i32, _i64, _p]),
     "ua_apply_cz": (_i32, [_p, _i32, _i32, _i32, _i32, _i64, _p]),
     "ua_apply_cnot": (_i32, [_p, _i32, _i32, _i32, _i32, _i64, _p]),
 }
@@ -81,6 +82,7 @@ UA_TUNE_UNROLL = 0
 UA_TUNE_LANE_EXCHANGE = 1
 UA_TUNE_MFMA = 2
 
+UA_PAULI_X = 0
 UA_PAULI_Y = 1
 UA_PAULI_Z = 2
 UA_MAX_GRAD_QUBITS = 16

@@ -1,29 +1,33 @@
-"""Adjoint gradients of batched hardware-efficient circuits.
+"""Adjoint gradients of batched rotation circuits with a diagonal observable.
 
-Circuit (per state b, n qubits, L layers), starting from |0...0>:
-    layer l:  RZ(phi[b,l,q]) then RY(theta[b,l,q]) on every qubit q,
-              then CZ on (q, q+1 mod n) for every q
-with RY(t) = exp(-i t/2 Y) = unitair exp_y(t/2) and RZ(t) = exp(-i t/2 Z) =
-exp_z(t/2) (unitair/gates/gates.py:35-91).  Loss per state: sum_i <Z_i> =
-diag_expectation_value(n - 2*popcount(x), psi) (unitair/states/innerprod.py:
-49-59).
+A circuit is a list of gates applied to every state b of a batch:
+    ("x" | "y" | "z", qubit, index[, coef])
+        unitair's exp_x / exp_y / exp_z(coef * params[b, index]) on `qubit`
+        (= exp(-i t P), unitair/gates/gates.py:7-91), applied as
+        apply_operator(gate, [qubit], state);
+    ("cz", a, b), ("cnot", control, target)   (gates.py:170-213);
+    ("dense", qubits, matrix)   a fixed 1- or 2-qubit gate (2x2 / 4x4).
+The loss per state is diag_expectation_value(observable, psi)
+(unitair/states/innerprod.py:49-59) for a real diagonal observable.
 
-unitair differentiates such a circuit with torch autograd through every
-apply_operator call, keeping one saved state per gate.  Here the backward pass
-keeps two states: psi, and the co-state lam = H psi carried back through the
-inverse gates.  For a rotation exp(-i t/2 P) the derivative is
-Im <lam|P|psi> at the point right after the gate; rotations of one kind on
-different qubits commute, so one (lam, psi) point per layer and kind gives all
-n derivatives (ua_pauli_gradients).  Un-applying gates in complex64 drifts, so
-psi is re-synchronised from checkpoints saved every ceil(sqrt(L)) layers.
-
-Forward and inverse layers run as fused passes (unitair_amd.fusion) with
-per-state gate data; the reductions run in float64.
+unitair differentiates such a loop of apply_operator calls with torch
+autograd, keeping one saved state per gate.  Here the backward pass keeps two
+states: psi, and the co-state lam = H psi carried back through the inverse
+gates.  For U(t) = exp(-i t P) the derivative of <psi|H|psi> is
+2 Im <lam|P|psi> at the point right after the gate.  Rotations on distinct
+qubits commute, so a run of them ("group") shares one (lam, psi) point, and
+one ua_pauli_gradients call per Pauli kind gives every qubit's value.  psi
+and lam are stored as one (2B, 2^n) batch, so every inverse step is one fused
+pass over both.  Un-applying gates in complex64 drifts, so psi is
+re-synchronised from checkpoints taken every ceil(sqrt(groups)) groups.
+Memory: two states plus ceil(sqrt(groups)) checkpoints.
 """
 from __future__ import annotations
 
 import math
+from typing import List, Optional, Sequence
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -31,76 +35,289 @@ from ._device import on_device
 from .fusion import FusedProgram, _Item
 
 _REAL = {torch.complex64: torch.float32, torch.complex128: torch.float64}
+_PAULI = {"x": _lib.UA_PAULI_X, "y": _lib.UA_PAULI_Y, "z": _lib.UA_PAULI_Z}
 
 
-def _rot_tensors(theta: torch.Tensor, phi: torch.Tensor, dtype: torch.dtype, sign: float):
-    """Per-state gate data, indexed [l][q] -> contiguous (B, 2, 2) RY and
-    (B, 2) RZ diagonal, for angles sign*theta / sign*phi."""
-    real = _REAL[dtype]
-    th = (sign * theta).to(real) / 2
-    ph = (sign * phi).to(real) / 2
-    c, s = torch.cos(th), torch.sin(th)
-    ry = torch.stack([torch.stack([c, -s], -1), torch.stack([s, c], -1)], -2).to(dtype)
-    e = torch.polar(torch.ones_like(ph), -ph).to(dtype)       # e^{-i phi/2}
-    rz = torch.stack([e, e.conj()], -1)
-    # (B, L, n, ...) -> (L, n, B, ...) contiguous
-    ry = ry.permute(1, 2, 0, 3, 4).contiguous()
-    rz = rz.permute(1, 2, 0, 3).contiguous()
-    return ry, rz
+def _rotation_data(kind: str, t: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
+    """Per-state gate data of exp(-i t P): (B, 2, 2) for X / Y, the (B, 2)
+    diagonal for Z (gates.py:7-91)."""
+    c, s = torch.cos(t), torch.sin(t)
+    if kind == "z":
+        return torch.stack([torch.complex(c, -s), torch.complex(c, s)], -1).to(dtype)
+    z = torch.zeros_like(c)
+    if kind == "x":
+        a, b = torch.complex(c, z), torch.complex(z, -s)
+        return torch.stack([torch.stack([a, b], -1), torch.stack([b, a], -1)], -2).to(dtype)
+    return torch.stack([torch.stack([c, -s], -1), torch.stack([s, c], -1)], -2).to(dtype)
 
 
-def _cz_ring(n):
-    if n < 2:
-        return []
-    pairs = [(q, (q + 1) % n) for q in range(n)] if n > 2 else [(0, 1)]
-    return [_Item("cz", (n - 1 - a, n - 1 - b)) for a, b in pairs]
+class _Spec:
+    """A gate list split into segments: (fixed gates, rotation group)."""
+
+    def __init__(self, gates: Sequence[tuple], n: int, num_params: int):
+        self.n = n
+        segments, fixed, rot = [], [], []
+        for g in gates:
+            kind = g[0]
+            if kind in _PAULI:
+                q, idx = int(g[1]), int(g[2])
+                coef = float(g[3]) if len(g) > 3 else 1.0
+                if not 0 <= q < n:
+                    raise ValueError(f"rotation qubit {q} out of range for {n} qubits")
+                if not 0 <= idx < num_params:
+                    raise ValueError(f"parameter index {idx} out of range ({num_params})")
+                if any(r[1] == q for r in rot):
+                    segments.append((fixed, rot))
+                    fixed, rot = [], []
+                rot.append((kind, q, idx, coef))
+            elif kind in ("cz", "cnot", "dense"):
+                if rot:
+                    segments.append((fixed, rot))
+                    fixed, rot = [], []
+                fixed.append(self._fixed_item(g))
+            else:
+                raise ValueError(f"unknown gate kind {kind!r}")
+        if rot:
+            segments.append((fixed, rot))
+            fixed = []
+        self.segments = segments
+        self.trailing = fixed
+
+    def _fixed_item(self, g):
+        n = self.n
+        if g[0] in ("cz", "cnot"):
+            a, b = int(g[1]), int(g[2])
+            if not (0 <= a < n and 0 <= b < n) or a == b:
+                raise ValueError(f"{g[0]} qubits ({a}, {b}) invalid for {n} qubits")
+            return _Item(g[0], (n - 1 - a, n - 1 - b))
+        qubits = tuple(int(q) for q in g[1])
+        m = np.asarray(g[2], dtype=np.complex128)
+        k = len(qubits)
+        if k not in (1, 2) or m.shape != (2 ** k, 2 ** k):
+            raise ValueError("fixed dense gates act on 1 or 2 qubits with a 2^k x 2^k matrix")
+        if len(set(qubits)) != k or not all(0 <= q < n for q in qubits):
+            raise ValueError(f"dense gate qubits {qubits} invalid for {n} qubits")
+        return _Item(f"dense{k}", tuple(n - 1 - q for q in qubits), m)
+
+
+def _inverse_fixed(it: _Item) -> _Item:
+    if it.kind in ("cz", "cnot"):
+        return it
+    return _Item(it.kind, it.bits, np.conj(it.data).T.copy())
+
+
+def _rot_items(rot, params, dtype, n, sign, repeat=1):
+    """Fused-pass items of a rotation group, angles sign * coef * params."""
+    items = []
+    for kind, q, idx, coef in rot:
+        t = (sign * coef) * params[:, idx]
+        data = _rotation_data(kind, t, dtype)
+        if repeat > 1:
+            data = data.repeat((repeat,) + (1,) * (data.dim() - 1))
+        data = data.contiguous()
+        items.append(_Item("diag1" if kind == "z" else "dense1", (n - 1 - q,), data, True))
+    return items
+
+
+def _program(items, n, dtype, device):
+    return FusedProgram.from_items(items, n, dtype, device) if items else None
+
+
+def _check_inputs(params, observable, n, dtype, device):
+    if dtype not in _REAL:
+        raise ValueError("adjoint circuits run complex64 or complex128 states")
+    if params.dim() != 2:
+        raise ValueError("params must have shape (batch, num_params)")
+    if device.type != "cuda":
+        raise RuntimeError("unitair_amd.adjoint runs on ROCm GPU tensors only")
+    if n < 4:
+        raise ValueError("the adjoint path needs at least 4 qubits")
+    if observable.shape[-1] != 2 ** n or observable.dim() not in (1, 2) or \
+            (observable.dim() == 2 and observable.shape[0] != params.shape[0]):
+        raise ValueError("observable must have shape (2^n,) or (batch, 2^n)")
+    if observable.is_complex():
+        raise ValueError("the observable is a real diagonal")
 
 
 class _Programs:
-    """Fused programs for every layer of one parameter set."""
+    """Every fused program of one parameter set, built before any of them
+    runs (building uploads op records, which waits for the stream)."""
 
-    def __init__(self, theta, phi, n, dtype, device):
-        L = theta.shape[1]
-        self.ry, self.rz = _rot_tensors(theta, phi, dtype, +1.0)
-        self.ry_inv, self.rz_inv = _rot_tensors(theta, phi, dtype, -1.0)
-        bit = lambda q: n - 1 - q
-        self.fwd, self.ry_back, self.rz_cz_back = [], [], []
-        for l in range(L):
-            items = [_Item("diag1", (bit(q),), self.rz[l, q], True) for q in range(n)]
-            items += [_Item("dense1", (bit(q),), self.ry[l, q], True) for q in range(n)]
-            items += _cz_ring(n)
-            self.fwd.append(FusedProgram.from_items(items, n, dtype, device))
-            self.ry_back.append(FusedProgram.from_items(
-                [_Item("dense1", (bit(q),), self.ry_inv[l, q], True) for q in range(n)],
-                n, dtype, device))
-            back = [_Item("diag1", (bit(q),), self.rz_inv[l, q], True) for q in range(n)]
-            if l > 0:
-                back += _cz_ring(n)          # the previous layer's CZ ring
-            self.rz_cz_back.append(FusedProgram.from_items(back, n, dtype, device))
-        self.cz_only = FusedProgram.from_items(_cz_ring(n), n, dtype, device) if n > 1 else None
-
-
-def _workspace(n, batch, device):
-    lib = _lib.load()
-    return torch.empty(lib.ua_pauli_gradients_workspace(n, batch), dtype=torch.uint8,
-                       device=device)
+    def __init__(self, spec, p, dtype, dev, backward: bool):
+        n = spec.n
+        S = len(spec.segments)
+        self.every = max(1, math.ceil(math.sqrt(max(S, 1))))
+        self.fwd = []
+        for c0 in range(0, S, self.every):
+            items = []
+            for fixed, rot in spec.segments[c0:c0 + self.every]:
+                items += list(fixed) + _rot_items(rot, p, dtype, n, +1.0)
+            self.fwd.append((c0, _program(items, n, dtype, dev)))
+        self.trail = _program(list(spec.trailing), n, dtype, dev)
+        self.inv_trail = self.inv = None
+        if backward:
+            self.inv_trail = _program([_inverse_fixed(it) for it in reversed(spec.trailing)],
+                                      n, dtype, dev)
+            self.inv = []
+            for fixed, rot in spec.segments:
+                items = _rot_items(rot, p, dtype, n, -1.0, repeat=2)
+                items += [_inverse_fixed(it) for it in reversed(fixed)]
+                self.inv.append(_program(items, n, dtype, dev))
 
 
-def _expect_sum_z(psi, n, ws):
-    lib = _lib.load()
-    out = torch.empty(psi.shape[0], dtype=torch.float64, device=psi.device)
-    rc = lib.ua_expect_sum_z(_lib.ptr(out), _lib.ptr(psi), _lib.DTYPE_CODES[psi.dtype], n,
-                             psi.shape[0], _lib.ptr(ws), _lib.stream_handle(psi.device))
-    _lib.check(rc, "expect_sum_z")
-    return out
+def _initial(B, n, dtype, dev, initial):
+    psi = torch.zeros(B, 2 ** n, dtype=dtype, device=dev)
+    if initial is None:
+        psi[:, 0] = 1
+    else:
+        psi.copy_(initial.detach().to(dev, dtype).expand(B, 2 ** n))
+    return psi
 
 
-def _pauli_grads(lam, psi, n, pauli, ws, out):
-    lib = _lib.load()
-    rc = lib.ua_pauli_gradients(_lib.ptr(out), _lib.ptr(lam), _lib.ptr(psi),
-                                _lib.DTYPE_CODES[psi.dtype], n, psi.shape[0], pauli,
-                                _lib.ptr(ws), _lib.stream_handle(psi.device))
-    _lib.check(rc, "pauli_gradients")
+class _AdjointExpectation(torch.autograd.Function):
+    """loss[b] = sum_x observable[(b,) x] |psi_b(x)|^2 (float64)."""
+
+    @staticmethod
+    def forward(ctx, params, observable, spec, dtype, initial):
+        n = spec.n
+        B = params.shape[0]
+        dev = params.device
+        p = params.detach().to(_REAL[dtype])
+        obs = observable.detach().to(dev, torch.float64).contiguous()
+        with on_device(dev):
+            progs = _Programs(spec, p, dtype, dev, backward=True)
+            psi = _initial(B, n, dtype, dev, initial)
+            ckpt = {}
+            for c0, prog in progs.fwd:
+                if c0 > 0:
+                    ckpt[c0] = psi.clone()          # state before segment c0
+                if prog is not None:
+                    prog.run(psi)
+            if progs.trail is not None:
+                progs.trail.run(psi)
+            loss = _diag_loss(obs, psi)
+        ctx.save_for_backward(params, observable)
+        ctx.state = (spec, dtype, obs, psi, ckpt, progs)
+        return loss
+
+    @staticmethod
+    def backward(ctx, gloss):
+        params, observable = ctx.saved_tensors
+        spec, dtype, obs, psi, ckpt, progs = ctx.state
+        ctx.state = None
+        n, B, dev = spec.n, params.shape[0], params.device
+        lib = _lib.load()
+        grad_p = grad_obs = None
+        with on_device(dev):
+            if ctx.needs_input_grad[1]:
+                from .states import abs_squared
+                w = gloss.to(torch.float64).view(B, 1) * abs_squared(psi).to(torch.float64)
+                grad_obs = (w if observable.dim() == 2 else w.sum(0)).to(observable.dtype)
+            if not ctx.needs_input_grad[0]:
+                return grad_p, grad_obs, None, None, None
+            X = torch.empty(2 * B, 2 ** n, dtype=dtype, device=dev)
+            X[:B].copy_(psi)
+            code = _lib.DTYPE_CODES
+            rc = lib.ua_apply_diagonal(_lib.ptr(X[B:]), _lib.ptr(psi), code[dtype],
+                                       _lib.ptr(obs), code[obs.dtype],
+                                       0 if obs.dim() == 1 else 2 ** n, n, B,
+                                       _lib.stream_handle(dev))
+            _lib.check(rc, "apply_diagonal")
+            del psi
+            if progs.inv_trail is not None:
+                progs.inv_trail.run(X)
+            ws = torch.empty(lib.ua_pauli_gradients_workspace(n, B), dtype=torch.uint8,
+                             device=dev)
+            vals = torch.empty(len(_PAULI), B, n, dtype=torch.float64, device=dev)
+            grad = torch.zeros(B, params.shape[1], dtype=torch.float64, device=dev)
+            for s in range(len(spec.segments) - 1, -1, -1):
+                fixed, rot = spec.segments[s]
+                for kind in sorted({r[0] for r in rot}):
+                    out = vals[_PAULI[kind]]
+                    rc = lib.ua_pauli_gradients(_lib.ptr(out), _lib.ptr(X[B:]), _lib.ptr(X[:B]),
+                                                code[dtype], n, B, _PAULI[kind], _lib.ptr(ws),
+                                                _lib.stream_handle(dev))
+                    _lib.check(rc, "pauli_gradients")
+                # one gather-add per group: grad[:, idx] += 2 coef vals[kind, :, q]
+                kinds = torch.tensor([_PAULI[r[0]] for r in rot], device=dev)
+                qs = torch.tensor([r[1] for r in rot], device=dev)
+                idx = torch.tensor([r[2] for r in rot], device=dev)
+                coef = torch.tensor([2.0 * r[3] for r in rot], dtype=torch.float64, device=dev)
+                grad.index_add_(1, idx, vals[kinds, :, qs].transpose(0, 1) * coef)
+                if progs.inv[s] is not None:
+                    progs.inv[s].run(X)
+                if s in ckpt:
+                    X[:B].copy_(ckpt[s])         # psi before segment s, as the forward had it
+            grad_p = (grad * gloss.to(torch.float64).view(B, 1)).to(params.dtype)
+        return grad_p, grad_obs, None, None, None
+
+
+def _diag_loss(obs: torch.Tensor, psi: torch.Tensor) -> torch.Tensor:
+    from .states import _reduce
+    return _reduce("diag", torch.float64, obs, psi)
+
+
+def expectation(params: torch.Tensor, gates: Sequence[tuple], observable: torch.Tensor,
+                num_qubits: int, dtype: torch.dtype = torch.complex64,
+                initial_state: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """loss[b] = diag_expectation_value(observable, U(params[b]) |psi0>) in
+    float64, differentiable in params (adjoint backward) and in the
+    observable.  params: (batch, num_params) real; gates: see the module
+    docstring; observable: (2^n,) or (batch, 2^n) real; initial_state:
+    |0...0> by default, else (2^n,) or (batch, 2^n) (not differentiated)."""
+    n = int(num_qubits)
+    if not isinstance(observable, torch.Tensor):
+        observable = torch.as_tensor(observable, device=params.device)
+    _check_inputs(params, observable, n, dtype, params.device)
+    spec = _Spec(gates, n, params.shape[1])
+    return _AdjointExpectation.apply(params, observable, spec, dtype, initial_state)
+
+
+def run_gates(params: torch.Tensor, gates: Sequence[tuple], num_qubits: int,
+              dtype: torch.dtype = torch.complex64,
+              initial_state: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Final states (batch, 2^n) of the circuit for every parameter row."""
+    n = int(num_qubits)
+    obs = torch.zeros(2 ** n, dtype=torch.float64, device=params.device)
+    _check_inputs(params, obs, n, dtype, params.device)
+    spec = _Spec(gates, n, params.shape[1])
+    B, dev = params.shape[0], params.device
+    p = params.detach().to(_REAL[dtype])
+    with on_device(dev):
+        items = []
+        for fixed, rot in spec.segments:
+            items += list(fixed) + _rot_items(rot, p, dtype, n, +1.0)
+        items += list(spec.trailing)
+        prog = _program(items, n, dtype, dev)
+        psi = _initial(B, n, dtype, dev, initial_state)
+        if prog is not None:
+            prog.run(psi)
+    return psi
+
+
+# ------------------------------------------------- hardware-efficient ansatz --
+
+def hardware_efficient_gates(num_qubits: int, layers: int) -> List[tuple]:
+    """Per layer l: RZ(phi[l, q]) then RY(theta[l, q]) on every qubit, then CZ
+    on (q, q+1 mod n); RY(t) = exp_y(t/2), RZ(t) = exp_z(t/2).  Parameter
+    index of theta[l, q] is l*2n + q, of phi[l, q] is l*2n + n + q."""
+    n = num_qubits
+    gates = []
+    pairs = [(q, (q + 1) % n) for q in range(n)] if n > 2 else [(0, 1)]
+    for l in range(layers):
+        gates += [("z", q, l * 2 * n + n + q, 0.5) for q in range(n)]
+        gates += [("y", q, l * 2 * n + q, 0.5) for q in range(n)]
+        gates += [("cz", a, b) for a, b in pairs]
+    return gates
+
+
+def sum_z_observable(num_qubits: int, device) -> torch.Tensor:
+    """sum_i Z_i as a diagonal: n - 2 popcount(x)."""
+    x = torch.arange(2 ** num_qubits, device=device, dtype=torch.int64)
+    pop = torch.zeros_like(x)
+    for b in range(num_qubits):
+        pop += (x >> b) & 1
+    return (num_qubits - 2 * pop).to(torch.float64)
 
 
 def _check(theta, phi, n):
@@ -112,82 +329,25 @@ def _check(theta, phi, n):
         raise ValueError("the adjoint path needs at least 4 qubits")
 
 
+def _ansatz_params(theta, phi):
+    B, L, n = theta.shape
+    return torch.cat([theta, phi], dim=2).reshape(B, L * 2 * n)
+
+
 def run_circuit(theta: torch.Tensor, phi: torch.Tensor, num_qubits: int,
                 dtype: torch.dtype = torch.complex64) -> torch.Tensor:
-    """Final states (batch, 2^n) of the circuit, no gradients."""
+    """Final states (batch, 2^n) of the hardware-efficient circuit."""
     _check(theta, phi, num_qubits)
-    B = theta.shape[0]
-    with on_device(theta.device):
-        progs = _Programs(theta.detach(), phi.detach(), num_qubits, dtype, theta.device)
-        psi = torch.zeros(B, 2 ** num_qubits, dtype=dtype, device=theta.device)
-        psi[:, 0] = 1
-        for prog in progs.fwd:
-            prog.run(psi)
-    return psi
-
-
-class AdjointSumZ(torch.autograd.Function):
-    """loss[b] = sum_i <Z_i> of the circuit's final state; float64."""
-
-    @staticmethod
-    def forward(ctx, theta, phi, num_qubits, dtype):
-        n = num_qubits
-        _check(theta, phi, n)
-        B, L = theta.shape[0], theta.shape[1]
-        dev = theta.device
-        with on_device(dev):
-            progs = _Programs(theta.detach(), phi.detach(), n, dtype, dev)
-            psi = torch.zeros(B, 2 ** n, dtype=dtype, device=dev)
-            psi[:, 0] = 1
-            every = max(1, math.ceil(math.sqrt(L)))
-            ckpt = {}
-            for l in range(L):
-                if l > 0 and l % every == 0:
-                    ckpt[l] = psi.clone()        # state before layer l
-                progs.fwd[l].run(psi)
-            ws = _workspace(n, B, dev)
-            loss = _expect_sum_z(psi, n, ws)
-        ctx.save_for_backward(theta, phi)
-        ctx.state = (progs, psi, ckpt, ws, n, dtype)
-        return loss
-
-    @staticmethod
-    def backward(ctx, gloss):
-        theta, phi = ctx.saved_tensors
-        progs, psi, ckpt, ws, n, dtype = ctx.state
-        ctx.state = None
-        B, L = theta.shape[0], theta.shape[1]
-        dev = theta.device
-        lib = _lib.load()
-        with on_device(dev):
-            lam = torch.empty_like(psi)
-            rc = lib.ua_apply_sum_z(_lib.ptr(lam), _lib.ptr(psi), _lib.DTYPE_CODES[dtype], n, B,
-                                    _lib.stream_handle(dev))
-            _lib.check(rc, "apply_sum_z")
-            gy = torch.empty(L, B, n, dtype=torch.float64, device=dev)
-            gz = torch.empty(L, B, n, dtype=torch.float64, device=dev)
-            if progs.cz_only is not None:
-                progs.cz_only.run(psi)
-                progs.cz_only.run(lam)
-            for l in range(L - 1, -1, -1):
-                _pauli_grads(lam, psi, n, _lib.UA_PAULI_Y, ws, gy[l])
-                progs.ry_back[l].run(psi)
-                progs.ry_back[l].run(lam)
-                _pauli_grads(lam, psi, n, _lib.UA_PAULI_Z, ws, gz[l])
-                progs.rz_cz_back[l].run(psi)
-                progs.rz_cz_back[l].run(lam)
-                if l in ckpt:
-                    # re-synchronise psi: the checkpoint is the state before
-                    # layer l; the previous layer's CZ ring is undone again
-                    psi.copy_(ckpt[l])
-                    progs.cz_only.run(psi)
-        scale = gloss.to(torch.float64).view(1, B, 1)
-        gtheta = (gy * scale).permute(1, 0, 2).to(theta.dtype)
-        gphi = (gz * scale).permute(1, 0, 2).to(phi.dtype)
-        return gtheta, gphi, None, None
+    return run_gates(_ansatz_params(theta, phi), hardware_efficient_gates(num_qubits,
+                                                                          theta.shape[1]),
+                     num_qubits, dtype)
 
 
 def sum_z_loss(theta: torch.Tensor, phi: torch.Tensor, num_qubits: int,
                dtype: torch.dtype = torch.complex64) -> torch.Tensor:
-    """Differentiable loss[b] = sum_i <Z_i> (float64) with adjoint gradients."""
-    return AdjointSumZ.apply(theta, phi, num_qubits, dtype)
+    """Differentiable loss[b] = sum_i <Z_i> (float64) of the hardware-efficient
+    circuit on (batch, layers, n) angle tensors, with adjoint gradients."""
+    _check(theta, phi, num_qubits)
+    n, L = num_qubits, theta.shape[1]
+    return expectation(_ansatz_params(theta, phi), hardware_efficient_gates(n, L),
+                       sum_z_observable(n, theta.device), n, dtype)

@@ -163,8 +163,43 @@ def _needs(it: _Item):
     return set()
 
 
-def plan_passes(items: Sequence[_Item], n: int, tile_bits: int = TILE_BITS):
-    """Greedy split of the item list into fused passes and wide gates."""
+_DIAG = ("phase", "cz", "diag1", "cz_layer")
+
+
+def _commute(a: _Item, b: _Item) -> bool:
+    """True when the two gates commute exactly (as operators), so either order
+    gives the same state: both diagonal; disjoint bits; a diagonal gate on a
+    CNOT's control only; two CNOTs neither of which targets the other's
+    control.  Full-length phases and CZ layers touch every bit."""
+    da, db = a.kind in _DIAG, b.kind in _DIAG
+    if da and db:
+        return True
+    if a.kind == "wide" or b.kind == "wide":
+        return False
+    ta = None if a.kind in ("phase", "cz_layer") else set(a.bits)
+    tb = None if b.kind in ("phase", "cz_layer") else set(b.bits)
+    if ta is None or tb is None:
+        return False
+    common = ta & tb
+    if not common:
+        return True
+    for x, y, dy in ((a, b, db), (b, a, da)):
+        if x.kind == "cnot" and dy and common <= {x.bits[0]}:
+            return True
+    if a.kind == "cnot" and b.kind == "cnot":
+        return a.bits[1] != b.bits[0] and b.bits[1] != a.bits[0]
+    return False
+
+
+def _predecessors(items: Sequence[_Item]):
+    """preds[j] = the earlier items that item j does not commute with."""
+    return [{i for i in range(j) if not _commute(items[i], items[j])}
+            for j in range(len(items))]
+
+
+def _plan_passes_in_order(items: Sequence[_Item], n: int, tile_bits: int = TILE_BITS):
+    """Greedy split of the item list, in its order, into fused passes and
+    wide gates."""
     m = min(n, tile_bits)
     base = set(range(min(n, CUBE_BITS)))
     out = []
@@ -202,10 +237,73 @@ def plan_passes(items: Sequence[_Item], n: int, tile_bits: int = TILE_BITS):
     return out
 
 
-_DIAG_KINDS = ("phase", "cz", "diag1", "cz_layer")
+def plan_passes(items: Sequence[_Item], n: int, tile_bits: int = TILE_BITS,
+                reorder: bool = True):
+    """Split the item list into fused passes and wide gates.
+
+    List scheduling over the commutation order: a pass keeps taking any gate
+    whose non-commuting predecessors are all placed and whose local bits
+    fit (gates needing no new local bit first), so gates that commute with a
+    pass-ending gate still join the pass.  Wide gates run alone, in order.
+    reorder=False keeps the item order (every amplitude then sees its gates
+    in the same order whatever the pass grouping, so results do not depend
+    on how a gate list is cut into programs)."""
+    if not reorder:
+        return _plan_passes_in_order(items, n, tile_bits)
+    m = min(n, tile_bits)
+    base = set(range(min(n, CUBE_BITS)))
+    items = list(items)
+    preds = _predecessors(items)
+    done = [False] * len(items)
+    left = len(items)
+    out = []
+    while left:
+        ready = [j for j in range(len(items)) if not done[j] and all(done[i] for i in preds[j])]
+        wide = [j for j in ready if items[j].kind == "wide"]
+        narrow = [j for j in ready if items[j].kind != "wide"]
+        if not narrow:
+            j = wide[0]
+            done[j] = True
+            left -= 1
+            out.append(items[j])
+            continue
+        local = set(base)
+        taken = []
+        while True:
+            best = None
+            for j in range(len(items)):
+                if done[j] or items[j].kind == "wide" or not all(done[i] for i in preds[j]):
+                    continue
+                grow = len(_needs(items[j]) - local)
+                if len(local) + grow > m:
+                    continue
+                if best is None or grow < best[0]:
+                    best = (grow, j)
+                    if grow == 0:
+                        break
+            if best is None:
+                break
+            j = best[1]
+            local |= _needs(items[j])
+            done[j] = True
+            left -= 1
+            taken.append(items[j])
+        out.append(_Pass(local=[], items=taken))
+    # pad each pass's local set to m bits and fix its order
+    for p in out:
+        if isinstance(p, _Pass):
+            loc = set(range(min(n, CUBE_BITS)))
+            for it in p.items:
+                loc |= _needs(it)
+            for bt in range(n):
+                if len(loc) >= m:
+                    break
+                loc.add(bt)
+            p.local = sorted(loc)
+    return out
 
 
-def _stages(p: _Pass):
+def _stages_in_order(p: _Pass):
     """Split a pass into stages: [(cube, [items])].
 
     cube is the sorted list of 4 tile bits of a cube stage, or None for a
@@ -220,7 +318,7 @@ def _stages(p: _Pass):
     stages = []
     cur = None                       # [cube set or None, items]
     for it in p.items:
-        if it.kind == "phase" or (it.kind in _DIAG_KINDS and cur is not None and
+        if it.kind == "phase" or (it.kind in _DIAG and cur is not None and
                                   cur[0] is None):
             if cur is None or cur[0] is not None:
                 if cur is not None:
@@ -252,6 +350,80 @@ def _stages(p: _Pass):
     return result
 
 
+def _stages(p: _Pass, reorder: bool = True):
+    """Split a pass into stages: [(cube, [items])], reordering gates only
+    across gates they commute with.
+
+    cube is the sorted list of 4 tile bits of a cube stage, or None for a
+    diagonal stage.  Full-length phases always run in diagonal stages (the
+    kernel reads their factors in the coalesced tile order), together with
+    every other diagonal gate that is ready then.  A cube stage starts from
+    the first ready dense gate or CNOT and keeps taking ready gates whose
+    target bits fit its 4 cube bits (diagonal gates other than phases need
+    none); a diagonal stage is made only when no such gate is ready.
+    reorder=False keeps the item order."""
+    if not reorder:
+        return _stages_in_order(p)
+    tile_of = {s: i for i, s in enumerate(p.local)}
+    m = len(p.local)
+    items = list(p.items)
+    preds = _predecessors(items)
+    done = [False] * len(items)
+    left = len(items)
+
+    def ready(j):
+        return not done[j] and all(done[i] for i in preds[j])
+
+    stages = []
+    while left:
+        cubeable = [j for j in range(len(items)) if ready(j) and items[j].kind != "phase"]
+        if cubeable:
+            cube, ops = set(), []
+            while True:
+                best = None
+                for j in range(len(items)):
+                    if not ready(j) or items[j].kind == "phase":
+                        continue
+                    grow = len({tile_of[b] for b in _needs(items[j])} - cube)
+                    if len(cube) + grow > CUBE_BITS:
+                        continue
+                    if best is None or grow < best[0]:
+                        best = (grow, j)
+                        if grow == 0:
+                            break
+                if best is None:
+                    break
+                j = best[1]
+                cube |= {tile_of[b] for b in _needs(items[j])}
+                done[j] = True
+                left -= 1
+                ops.append(items[j])
+            stages.append((cube, ops))
+            continue
+        ops = []
+        while True:
+            diag = [j for j in range(len(items)) if ready(j) and items[j].kind in _DIAG]
+            if not diag:
+                break
+            for j in diag:
+                done[j] = True
+                left -= 1
+                ops.append(items[j])
+        stages.append((None, ops))
+    result = []
+    for cube, ops in stages:
+        if cube is None:
+            result.append((None, ops))
+            continue
+        cube = set(cube)
+        for t in range(m):
+            if len(cube) >= CUBE_BITS:
+                break
+            cube.add(t)
+        result.append((sorted(cube), ops))
+    return result
+
+
 class FusedProgram:
     """A circuit compiled into fused passes for one (n, dtype, device).
 
@@ -259,22 +431,24 @@ class FusedProgram:
     be per state (a device tensor with a leading batch dimension)."""
 
     def __init__(self, circuit: Circuit, dtype: torch.dtype = torch.complex64,
-                 device="cuda", tile_bits: int = TILE_BITS, merge_1q: bool = True):
+                 device="cuda", tile_bits: int = TILE_BITS, merge_1q: bool = True,
+                 reorder: bool = True):
         items = _items_from_circuit(circuit)
         if merge_1q:
             items = _merge_1q(items)
-        self._build(items, circuit.num_qubits, dtype, device, tile_bits)
+        self._build(items, circuit.num_qubits, dtype, device, tile_bits, reorder)
         self.num_gates = len(circuit.gates)
 
     @classmethod
     def from_items(cls, items: List[_Item], n: int, dtype: torch.dtype = torch.complex64,
-                   device="cuda", tile_bits: int = TILE_BITS) -> "FusedProgram":
+                   device="cuda", tile_bits: int = TILE_BITS,
+                   reorder: bool = True) -> "FusedProgram":
         prog = cls.__new__(cls)
-        prog._build(list(items), n, dtype, device, tile_bits)
+        prog._build(list(items), n, dtype, device, tile_bits, reorder)
         prog.num_gates = len(items)
         return prog
 
-    def _build(self, items, n, dtype, device, tile_bits):
+    def _build(self, items, n, dtype, device, tile_bits, reorder=True):
         if dtype not in (torch.complex64, torch.complex128):
             raise ValueError("fused programs run complex64/complex128 states")
         for it in items:
@@ -290,7 +464,7 @@ class FusedProgram:
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         real = torch.float32 if dtype == torch.complex64 else torch.float64
-        plan = plan_passes(items, self.n, tile_bits)
+        plan = plan_passes(items, self.n, tile_bits, reorder)
         self._keep = []
         self.steps = []          # ("fused", FusedPass, ops_ptr, nops) | ("wide", qubits, mat)
         mats, op_structs = [], []
@@ -302,7 +476,7 @@ class FusedProgram:
                 qubits = [self.n - 1 - b for b in p.bits]
                 self.steps.append(("wide", qubits, m))
                 continue
-            stages = _stages(p)
+            stages = _stages(p, reorder)
             for chunk in range(0, len(stages), MAX_STAGES):
                 part = stages[chunk:chunk + MAX_STAGES]
                 fp = FusedPass()

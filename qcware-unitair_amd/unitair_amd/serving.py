@@ -155,7 +155,11 @@ class _Lane:
                 for q in range(self.n):
                     items.append(_Item("dense1", (self.n - 1 - q,), self.mats[l, q], True))
                 items.append(_Item("cz_layer", (), self.masks[l], True))
-            prog = FusedProgram.from_items(items, self.n, torch.complex64, self.device)
+            # in order: a request's gates meet each amplitude in the same
+            # order whatever K and the other slots, so its result is the same
+            # alone and in the mix
+            prog = FusedProgram.from_items(items, self.n, torch.complex64, self.device,
+                                           reorder=False)
             # data_bstride for per-slot matrices: consecutive slots are 4 (2x2)
             # elements apart inside mats[l, q] (shape (capacity, 2, 2)), and 1
             # mask apart inside masks[l] -- both match the defaults.

@@ -41,6 +41,8 @@ def items_to_gates(items, n):
         qubits = tuple(n - 1 - b for b in it.bits)
         if it.kind in ("dense1", "dense2", "wide"):
             gates.append(Gate("dense", qubits, matrix=it.data))
+        elif it.kind == "diag1":
+            gates.append(Gate("dense", qubits, matrix=np.diag(it.data)))
         elif it.kind == "phase":
             gates.append(Gate("phase", (), angles=it.data))
         else:
@@ -91,9 +93,59 @@ def test_plan_respects_limits(n, seed):
             for it in ops:
                 assert {tile_of[b] for b in fusion._needs(it)} <= set(cube)
             seen += ops
-        assert seen == p.items
-        flat += p.items
-    assert [id(x) for x in flat] == [id(x) for x in items]      # order and count kept
+        assert sorted(map(id, seen)) == sorted(map(id, p.items))
+        flat += seen
+    # every gate once, and any two gates that do not commute keep their order
+    assert sorted(map(id, flat)) == sorted(map(id, items))
+    pos = {id(x): i for i, x in enumerate(flat)}
+    for i in range(len(items)):
+        for j in range(i + 1, len(items)):
+            if not fusion._commute(items[i], items[j]):
+                assert pos[id(items[i])] < pos[id(items[j])]
+    # and the reordered circuit gives the same state (oracle, complex128)
+    rng = np.random.default_rng(seed + 7)
+    psi = rng.standard_normal(2 ** n) + 1j * rng.standard_normal(2 ** n)
+    a = oc.run_closed_form(items_to_gates(items, n), psi)
+    b = oc.run_closed_form(items_to_gates(flat, n), psi)
+    assert np.allclose(a, b, atol=1e-11)
+
+
+def test_commutation_rule_against_matrices():
+    """fusion._commute claims exact commutation; check each claim (and each
+    refusal that matters) with explicit operators on 4 qubits."""
+    from unitair_amd.fusion import _Item, _commute
+    rng = np.random.default_rng(3)
+    n = 4
+    H = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+
+    def full(it):
+        psi_cols = []
+        for x in range(2 ** n):
+            e = np.zeros(2 ** n, dtype=np.complex128)
+            e[x] = 1
+            psi_cols.append(oc.run_closed_form(items_to_gates([it], n), e))
+        return np.stack(psi_cols, axis=1)
+
+    kinds = []
+    for b in range(n):
+        kinds.append(_Item("dense1", (b,), H))
+        kinds.append(_Item("diag1", (b,), np.exp(1j * rng.random(2))))
+    for a in range(n):
+        for b in range(n):
+            if a != b:
+                kinds += [_Item("cz", (a, b)), _Item("cnot", (a, b)),
+                          _Item("dense2", (a, b), np.linalg.qr(
+                              rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0])]
+    kinds.append(_Item("phase", (), rng.uniform(0, 6, 2 ** n)))
+    mats = [full(k) for k in kinds]
+    for i, a in enumerate(kinds):
+        for j, b in enumerate(kinds):
+            if _commute(a, b):
+                assert np.allclose(mats[i] @ mats[j], mats[j] @ mats[i], atol=1e-12), \
+                    (a.kind, a.bits, b.kind, b.bits)
+    # CNOTs sharing a control commute; a CNOT and a gate on its target do not
+    assert _commute(_Item("cnot", (0, 1)), _Item("cnot", (0, 2)))
+    assert not _commute(_Item("cnot", (0, 1)), _Item("dense1", (1,), H))
 
 
 def test_24q_circuit_needs_few_passes():
@@ -166,3 +218,23 @@ def test_dense_gates_wider_than_the_in_place_limit_are_rejected_at_build():
     items = [fusion._Item("wide", tuple(range(6)), haar_unitary(64, rng))]
     with pytest.raises(ValueError, match="1..5"):
         FusedProgram.from_items(items, 8, dtype=torch.complex64, device="cpu")
+
+
+@pytest.mark.parametrize("n,seed", [(7, 1), (16, 3)])
+def test_in_order_plan_keeps_the_gate_order(n, seed):
+    """reorder=False (used by the serving loop, whose results must not
+    depend on how a request's layers are cut into programs) keeps every gate
+    in its place."""
+    circ = random_mixed_circuit(n, 120, seed)
+    items = fusion._merge_1q(fusion._items_from_circuit(circ))
+    flat = []
+    for p in fusion.plan_passes(items, n, reorder=False):
+        if isinstance(p, fusion._Item):
+            flat.append(p)
+            continue
+        seen = []
+        for cube, ops in fusion._stages(p, reorder=False):
+            seen += ops
+        assert seen == p.items
+        flat += p.items
+    assert [id(x) for x in flat] == [id(x) for x in items]

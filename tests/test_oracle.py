@@ -484,3 +484,51 @@ def test_pins_detect_planted_mistakes(mutant):
 
 def test_pins_detect_phase_sign_mistake():
     assert doc_failures(U.apply_operator, phase_fn=lambda a, s: np.exp(1j * a) * s)
+
+
+def test_general_rotation_oracle_pins():
+    """Pins of the general-adjoint oracle (tests/adjoint_oracle.py):
+    exp_pauli(t) is expm(-i t P) for X, Y, Z (gates.py:7-91 define exp_x/y/z
+    as e^(-i angle P)); the parameter-shift rule with shared parameters and
+    coefficients equals central differences of the loss; and a planted wrong
+    shift (pi/2 instead of pi/4) fails."""
+    import scipy.linalg
+    import adjoint_oracle as AO
+    P = {"x": np.array([[0, 1], [1, 0]]), "y": np.array([[0, -1j], [1j, 0]]),
+         "z": np.diag([1, -1])}
+    for k, m in P.items():
+        for t in (0.3, -1.7, 2.9):
+            assert np.allclose(AO.exp_pauli(k, t), scipy.linalg.expm(-1j * t * m), atol=1e-14)
+    rng = np.random.default_rng(5)
+    n, num = 4, 5
+    gates = AO.random_rotation_circuit(rng, n, 25, num)
+    params = rng.uniform(-np.pi, np.pi, num)
+    obs = rng.standard_normal(2 ** n)
+    h = 1e-5
+    for i in range(num):
+        pp, pm = params.copy(), params.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (AO.general_loss(pp, gates, obs, n) - AO.general_loss(pm, gates, obs, n)) / (2 * h)
+        assert abs(AO.general_param_shift(params, gates, obs, n, i) - fd) < 1e-7
+    # planted mistake: the frequency-1 shift (pi/2, halved) is wrong for exp(-i t P)
+    i = next(g[2] for g in gates if g[0] in ("x", "y"))
+    g0 = next(gi for gi, g in enumerate(gates) if g[0] in ("x", "y") and g[2] == i)
+    def wrong(params):
+        tot = 0.0
+        for gi, g in enumerate(gates):
+            if g[0] in ("x", "y", "z") and g[2] == i:
+                coef = g[3]
+                vals = []
+                for d in (np.pi / 2, -np.pi / 2):
+                    gs = list(gates)
+                    gs[gi] = (g[0], g[1], len(params), 1.0)
+                    vals.append(AO.general_loss(np.append(params, coef * params[i] + d), gs,
+                                                obs, n))
+                tot += coef * 0.5 * (vals[0] - vals[1])
+        return tot
+    pp, pm = params.copy(), params.copy()
+    pp[i] += h
+    pm[i] -= h
+    fd = (AO.general_loss(pp, gates, obs, n) - AO.general_loss(pm, gates, obs, n)) / (2 * h)
+    assert abs(wrong(params) - fd) > 1e-4
