@@ -302,35 +302,64 @@ def adjoint_bench(dev, iters=3):
             "speedup_states_per_s": (B / adj_s) / (Bp / plain_s)}
 
 
-def serving_bench(dev, num=24, seed=11):
+def serving_bench(dev, num=200, seed=11, cap_sample=40):
     """Poisson arrivals of random layered circuits (n log-uniform in [10, 26],
-    depth uniform in [10, 200], complex64).  The arrival rate is 0.7 x the
-    one-at-a-time capacity measured on the same requests; the continuous-
-    batching server and the one-at-a-time plain path see identical arrivals."""
-    from unitair_amd.serving import Server, poisson_workload, run_one_at_a_time, summarize
-    warm = poisson_workload(4, rate=0.0, seed=seed + 1, n_range=(10, 14), depth_range=(10, 20))
-    Server(device=dev).run(warm, realtime=False)
-    run_one_at_a_time(poisson_workload(4, 0.0, seed + 1, (10, 14), (10, 20)), device=dev,
-                      realtime=False)
-    cap_reqs = poisson_workload(num, rate=0.0, seed=seed)
+    depth uniform in [10, 200], complex64), `num` requests.  The arrival rate
+    is 0.7 x the one-at-a-time per-gate capacity (measured on the first
+    `cap_sample` requests of the same stream).  The continuous-batching server
+    and two one-at-a-time baselines -- per-gate launches, and the server's own
+    fused programs with one request on the GPU -- see identical arrivals."""
+    from unitair_amd.serving import (Server, poisson_workload, run_one_at_a_time,
+                                     run_one_at_a_time_fused, summarize)
+    warm = lambda: poisson_workload(4, rate=0.0, seed=seed + 1, n_range=(10, 14),
+                                    depth_range=(10, 20))
+    Server(device=dev).run(warm(), realtime=False)
+    run_one_at_a_time(warm(), device=dev, realtime=False)
+    run_one_at_a_time_fused(warm(), device=dev, realtime=False)
+    cap_reqs = poisson_workload(num, rate=0.0, seed=seed)[:cap_sample]
     run_one_at_a_time(cap_reqs, device=dev, realtime=False)
-    capacity = num / max(r.done for r in cap_reqs)
-    # the same requests all at once through the continuous-batching server
-    sat_reqs = poisson_workload(num, rate=0.0, seed=seed)
-    Server(device=dev).run(sat_reqs, realtime=False)
-    capacity_cb = num / max(r.done for r in sat_reqs)
+    capacity = len(cap_reqs) / max(r.done for r in cap_reqs)
+    sat = poisson_workload(num, rate=0.0, seed=seed)
+    Server(device=dev).run(sat, realtime=False)
+    capacity_cb = num / max(r.done for r in sat)
+    sat1 = poisson_workload(num, rate=0.0, seed=seed)
+    run_one_at_a_time_fused(sat1, device=dev, realtime=False)
+    capacity_f1 = num / max(r.done for r in sat1)
     rate = 0.7 * capacity
+    log(f"serving: per-gate capacity {capacity:.1f}/s, arrivals at {rate:.1f}/s")
     served = Server(device=dev).run(poisson_workload(num, rate=rate, seed=seed), realtime=True)
+    fused1 = run_one_at_a_time_fused(poisson_workload(num, rate=rate, seed=seed), device=dev,
+                                     realtime=True)
     plain = run_one_at_a_time(poisson_workload(num, rate=rate, seed=seed), device=dev,
                               realtime=True)
-    same = max(max(abs(a.z0 - b.z0), abs(a.norm2 - b.norm2)) for a, b in zip(served, plain))
+    # heavier load: 0.9 x the capacity of one request at a time on fused programs
+    rate_hi = 0.9 * capacity_f1
+    log(f"serving: fused one-at-a-time capacity {capacity_f1:.1f}/s, arrivals at {rate_hi:.1f}/s")
+    served_hi = Server(device=dev).run(poisson_workload(num, rate=rate_hi, seed=seed),
+                                       realtime=True)
+    fused1_hi = run_one_at_a_time_fused(poisson_workload(num, rate=rate_hi, seed=seed),
+                                        device=dev, realtime=True)
+    same_fused = max(max(abs(a.z0 - b.z0), abs(a.norm2 - b.norm2))
+                     for a, b in zip(served, fused1))
+    same_plain = max(max(abs(a.z0 - b.z0), abs(a.norm2 - b.norm2))
+                     for a, b in zip(served, plain))
     return {"workload": f"{num} requests, n log-uniform [10,26], depth U[10,200] layers of "
                         f"Haar 1q on every qubit + CZ on n//2 adjacent pairs, complex64",
             "one_at_a_time_capacity_circuits_per_s": capacity,
+            "capacity_sample_requests": len(cap_reqs),
             "continuous_batching_capacity_circuits_per_s": capacity_cb,
-            "capacity_ratio": capacity_cb / capacity, "arrival_rate_per_s": rate,
-            "continuous_batching": summarize(served), "one_at_a_time": summarize(plain),
-            "max_result_difference": same}
+            "one_at_a_time_fused_capacity_circuits_per_s": capacity_f1,
+            "capacity_ratio": capacity_cb / capacity,
+            "capacity_ratio_vs_fused_one_at_a_time": capacity_cb / capacity_f1,
+            "arrival_rate_per_s": rate,
+            "continuous_batching": summarize(served),
+            "one_at_a_time_fused_programs": summarize(fused1),
+            "one_at_a_time_per_gate": summarize(plain),
+            "heavy_load": {"arrival_rate_per_s": rate_hi,
+                           "continuous_batching": summarize(served_hi),
+                           "one_at_a_time_fused_programs": summarize(fused1_hi)},
+            "max_difference_vs_fused_alone": same_fused,
+            "max_difference_vs_per_gate": same_plain}
 
 
 MFMA_PEAK_TFLOPS = {"c64": 157.3, "c128": 78.6}   # dense FP32 / FP64 matrix-core peaks
@@ -701,11 +730,12 @@ def main():
         log("serving workload")
         extra["serving"] = serving_bench(dev)
         sv = extra["serving"]
-        log(f"serving: continuous p50 {sv['continuous_batching']['p50_ms']:.1f} ms p99 "
-            f"{sv['continuous_batching']['p99_ms']:.1f} ms, "
-            f"{sv['continuous_batching']['circuits_per_s']:.1f} circ/s; one-at-a-time p50 "
-            f"{sv['one_at_a_time']['p50_ms']:.1f} p99 {sv['one_at_a_time']['p99_ms']:.1f} ms, "
-            f"{sv['one_at_a_time']['circuits_per_s']:.1f} circ/s")
+        for tag, blk in (("", sv), ("heavy load ", sv["heavy_load"])):
+            for key in ("continuous_batching", "one_at_a_time_fused_programs",
+                        "one_at_a_time_per_gate"):
+                if key in blk:
+                    log(f"serving {tag}{key}: p50 {blk[key]['p50_ms']:.1f} ms p99 "
+                        f"{blk[key]['p99_ms']:.1f} ms, {blk[key]['circuits_per_s']:.1f} circ/s")
 
     if not args.no_stream_ref:
         extra["stream_reference"] = stream_reference(dev, n)

@@ -1,6 +1,8 @@
-"""Minimal ctypes access to the HIP runtime torch already loaded (events only).
+"""Minimal ctypes access to the HIP runtime torch already loaded.
 
-Used to hand hipEvent_t handles to ua_run_circuit for per-gate timing.
+Events: hipEvent_t handles for ua_run_circuit's per-gate timing.
+StreamGraph: a HIP graph recorded by stream capture and replayed on a
+stream, without the device-wide synchronisation torch's graph capture does.
 """
 from __future__ import annotations
 
@@ -24,6 +26,19 @@ def runtime() -> ctypes.CDLL:
         rt.hipEventElapsedTime.restype = ctypes.c_int
         rt.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         rt.hipEventSynchronize.restype = ctypes.c_int
+        rt.hipStreamBeginCapture.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        rt.hipStreamBeginCapture.restype = ctypes.c_int
+        rt.hipStreamEndCapture.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        rt.hipStreamEndCapture.restype = ctypes.c_int
+        rt.hipGraphInstantiate.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        rt.hipGraphInstantiate.restype = ctypes.c_int
+        rt.hipGraphLaunch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        rt.hipGraphLaunch.restype = ctypes.c_int
+        rt.hipGraphExecDestroy.argtypes = [ctypes.c_void_p]
+        rt.hipGraphExecDestroy.restype = ctypes.c_int
+        rt.hipGraphDestroy.argtypes = [ctypes.c_void_p]
+        rt.hipGraphDestroy.restype = ctypes.c_int
         _rt = rt
     return _rt
 
@@ -68,3 +83,44 @@ class EventPool:
             for h in self._made:
                 _rt.hipEventDestroy(ctypes.c_void_p(h))
             self._alive = False
+
+
+_CAPTURE_RELAXED = 2   # hipStreamCaptureModeRelaxed
+
+
+def _handle(stream) -> ctypes.c_void_p:
+    return stream if isinstance(stream, ctypes.c_void_p) else ctypes.c_void_p(stream)
+
+
+class StreamGraph:
+    """Records fn() -- kernel launches on `stream` (a hipStream_t handle) and
+    nothing that allocates, synchronises or sets memory -- into a HIP graph by
+    stream capture; launch() replays it on a stream."""
+
+    def __init__(self, stream, fn):
+        rt = runtime()
+        stream = _handle(stream)
+        self._graph = ctypes.c_void_p()
+        self._exec = ctypes.c_void_p()
+        if rt.hipStreamBeginCapture(stream, _CAPTURE_RELAXED) != 0:
+            raise RuntimeError("hipStreamBeginCapture failed")
+        try:
+            fn()
+        finally:
+            rc = rt.hipStreamEndCapture(stream, ctypes.byref(self._graph))
+        if rc != 0:
+            raise RuntimeError("hipStreamEndCapture failed")
+        if rt.hipGraphInstantiate(ctypes.byref(self._exec), self._graph, None, None, 0) != 0:
+            rt.hipGraphDestroy(self._graph)
+            raise RuntimeError("hipGraphInstantiate failed")
+
+    def launch(self, stream):
+        if runtime().hipGraphLaunch(self._exec, _handle(stream)) != 0:
+            raise RuntimeError("hipGraphLaunch failed")
+
+    def __del__(self):
+        if _rt is not None:
+            if getattr(self, "_exec", None) and self._exec.value:
+                _rt.hipGraphExecDestroy(self._exec)
+            if getattr(self, "_graph", None) and self._graph.value:
+                _rt.hipGraphDestroy(self._graph)

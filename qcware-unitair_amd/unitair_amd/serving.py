@@ -11,16 +11,21 @@ Design (one GPU):
     `capacity` state slots (capacity x 2^n x 8 B).  A request is admitted
     when its lane has a free slot, or when a new lane fits the HBM budget;
     otherwise it waits in the queue.  Lanes are released when they empty.
-  * Continuous batching.  Every scheduler step advances each lane by
+  * Continuous batching.  Each lane step advances the lane by
     K = min(max_layers_per_step, layers left of its least advanced request)
     layers, with one batched fused-pass program over the lane's occupied
     slot range: per-slot gate data (each slot is at its own layer), so
-    requests that arrived later join the next step of their lane.  Empty
-    slots inside the range run identity gates.
-  * Overlap.  Each lane runs on its own HIP stream, so lanes of different n
-    overlap on the device; the step's gate data is staged once per lane.
-  * Graph replay.  Each (lane, K, occupied range) program is captured into a
-    HIP graph on first use and replayed afterwards.
+    requests that arrive later join the lane's next step.  Empty slots inside
+    the range run identity gates.
+  * Asynchronous.  Each lane runs on its own HIP stream with up to two steps
+    in flight (double-buffered gate staging).  The host never synchronises
+    the device: a step ends with an event, and the scheduler polls the
+    events, harvests finished requests (their reductions were queued right
+    after their last step), admits waiting requests into freed slots and
+    issues further steps while other lanes keep the GPU busy.  A request's
+    completion time is its step's event time.
+  * Graph replay.  Each (lane, K, occupied range, staging buffer) program is
+    recorded into a HIP graph by stream capture on first use and replayed.
 Each request's arithmetic is independent of its slot and of the other
 requests, so its result is the same alone and in the mix.
 """
@@ -28,6 +33,7 @@ from __future__ import annotations
 
 import math
 import time
+from collections import deque
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
 
@@ -36,6 +42,7 @@ import torch
 
 from . import _lib
 from ._device import on_device
+from ._hip import StreamGraph
 from .fusion import FusedProgram, _Item
 from .gates import haar_unitary
 
@@ -116,23 +123,33 @@ def _z0_diag(n: int, device) -> torch.Tensor:
 
 
 class _Lane:
-    def __init__(self, n: int, capacity: int, kmax: int, device):
+    NBUF = 2   # steps in flight per lane (gate staging buffers)
+
+    def __init__(self, n: int, capacity: int, kmax: int, device, use_graph: bool):
         self.n = n
         self.capacity = capacity
         self.kmax = kmax
         self.device = device
+        self.use_graph = use_graph
         self.states = torch.empty(capacity, 2 ** n, dtype=torch.complex64, device=device)
-        self.mats = torch.empty(kmax, n, capacity, 2, 2, dtype=torch.complex64, device=device)
-        self.masks = torch.zeros(kmax, capacity, dtype=torch.int64, device=device)
-        self.h_mats = torch.empty(self.mats.shape, dtype=torch.complex64, pin_memory=True)
-        self.h_masks = torch.zeros(self.masks.shape, dtype=torch.int64, pin_memory=True)
+        shape_m, shape_k = (kmax, n, capacity, 2, 2), (kmax, capacity)
+        self.mats = [torch.empty(shape_m, dtype=torch.complex64, device=device)
+                     for _ in range(self.NBUF)]
+        self.masks = [torch.zeros(shape_k, dtype=torch.int64, device=device)
+                      for _ in range(self.NBUF)]
+        self.h_mats = [torch.empty(shape_m, dtype=torch.complex64, pin_memory=True)
+                       for _ in range(self.NBUF)]
+        self.h_masks = [torch.zeros(shape_k, dtype=torch.int64, pin_memory=True)
+                        for _ in range(self.NBUF)]
         self.slots: List[Optional[Request]] = [None] * capacity
-        self.pos = [0] * capacity           # next layer of each slot
+        self.pos = [0] * capacity           # next layer of each slot (as issued)
         self.data: Dict[int, tuple] = {}    # rid -> (mats, masks) host arrays
         self.stream = torch.cuda.Stream(device)   # default priority
-        self.programs: Dict[tuple, tuple] = {}
+        self.programs: Dict[tuple, object] = {}
         self.diag = _z0_diag(n, device)
         self.identity = np.eye(2, dtype=np.complex64)
+        self.inflight = deque()             # (event, [(request, norm, z0)])
+        self.next_buf = 0
 
     @property
     def bytes(self) -> int:
@@ -147,76 +164,109 @@ class _Lane:
     def active(self):
         return [i for i, r in enumerate(self.slots) if r is not None]
 
-    def _program(self, K: int, used: int, use_graph: bool):
-        key = (K, used)
+    def idle(self) -> bool:
+        return not self.active() and not self.inflight
+
+    def _program(self, K: int, used: int, buf: int):
+        """FusedProgram (or its graph) for K layers over slots [0, used)
+        reading staging buffer `buf`."""
+        key = (K, used, buf)
         if key not in self.programs:
             items = []
             for l in range(K):
                 for q in range(self.n):
-                    items.append(_Item("dense1", (self.n - 1 - q,), self.mats[l, q], True))
-                items.append(_Item("cz_layer", (), self.masks[l], True))
+                    items.append(_Item("dense1", (self.n - 1 - q,), self.mats[buf][l, q], True))
+                items.append(_Item("cz_layer", (), self.masks[buf][l], True))
+            # built on the default stream: its op-record upload waits only
+            # for that stream, not for the lane's queued steps
             # in order: a request's gates meet each amplitude in the same
             # order whatever K and the other slots, so its result is the same
             # alone and in the mix
             prog = FusedProgram.from_items(items, self.n, torch.complex64, self.device,
                                            reorder=False)
-            # data_bstride for per-slot matrices: consecutive slots are 4 (2x2)
-            # elements apart inside mats[l, q] (shape (capacity, 2, 2)), and 1
-            # mask apart inside masks[l] -- both match the defaults.
-            graph = None
-            if use_graph:
-                view = self.states[:used]
-                with torch.cuda.stream(self.stream):
-                    prog.capture(view)
-                graph = prog
-            self.programs[key] = (prog, graph)
+            # per-slot data: consecutive slots are 4 (2x2) elements apart in
+            # mats[buf][l, q] and 1 mask apart in masks[buf][l], the defaults
+            view = self.states[:used]
+            entry = prog
+            if self.use_graph:
+                with torch.cuda.stream(self.stream):   # capture records, runs nothing
+                    handle = _lib.stream_handle(self.device)
+                    entry = (prog, StreamGraph(handle, lambda: prog.run(view)))
+            self.programs[key] = entry
         return self.programs[key]
 
-    def step(self, use_graph: bool):
+    def issue(self):
+        """Queue the lane's next step (gate staging, the fused passes, the
+        reductions of the requests it completes, an event) on the lane's
+        stream.  Slots of completed requests are free for admission at once:
+        later work on the stream is ordered after their reductions."""
         act = self.active()
         used = max(act) + 1
         K = min(self.kmax, min(self.slots[i].depth - self.pos[i] for i in act))
+        b = self.next_buf
+        self.next_buf = (b + 1) % self.NBUF
+        hm, hk = self.h_mats[b], self.h_masks[b]
         for i in range(used):
             r = self.slots[i]
             if r is None:
-                self.h_mats[:K, :, i] = torch.from_numpy(self.identity)
-                self.h_masks[:K, i] = 0
+                hm[:K, :, i] = torch.from_numpy(self.identity)
+                hk[:K, i] = 0
                 continue
             mats, masks = self.data[r.rid]
             p = self.pos[i]
-            self.h_mats[:K, :, i] = torch.from_numpy(mats[p:p + K])
-            self.h_masks[:K, i] = torch.from_numpy(masks[p:p + K])
+            hm[:K, :, i] = torch.from_numpy(mats[p:p + K])
+            hk[:K, i] = torch.from_numpy(masks[p:p + K])
+        results = []
+        entry = self._program(K, used, b)
         with torch.cuda.stream(self.stream):
-            self.mats.copy_(self.h_mats, non_blocking=True)
-            self.masks.copy_(self.h_masks, non_blocking=True)
-            prog, graph = self._program(K, used, use_graph)
-            if graph is not None:
-                graph.replay()
+            self.mats[b].copy_(hm, non_blocking=True)
+            self.masks[b].copy_(hk, non_blocking=True)
+            if isinstance(entry, tuple):
+                entry[1].launch(_lib.stream_handle(self.device))
             else:
-                prog.run(self.states[:used])
-        for i in act:
-            self.pos[i] += K
-        return K
+                entry.run(self.states[:used])
+            for i in act:
+                self.pos[i] += K
+                r = self.slots[i]
+                if self.pos[i] >= r.depth:
+                    results.append((r,) + self.collect(i))
+                    self.slots[i] = None
+                    del self.data[r.rid]
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(self.stream)
+        self.inflight.append((ev, results))
 
-    def finished(self):
-        return [i for i in self.active() if self.pos[i] >= self.slots[i].depth]
+    def harvest(self, start_event):
+        """Requests of completed steps, with their completion times (seconds
+        after start_event) and results."""
+        out = []
+        while self.inflight and self.inflight[0][0].query():
+            ev, results = self.inflight.popleft()
+            t = start_event.elapsed_time(ev) / 1e3
+            for r, norm, z0 in results:
+                r.norm2 = float(norm)
+                r.z0 = float(z0)
+                r.done = t
+                out.append(r)
+        return out
 
     def collect(self, i: int):
-        """float64 (||psi||^2, <Z_0>) of slot i, computed on the lane stream."""
+        """float64 (||psi||^2, <Z_0>) of slot i, queued on the lane stream."""
         from .states import _reduce
-        with torch.cuda.stream(self.stream):
-            row = self.states[i:i + 1]
-            norm = _reduce("norm", torch.float64, row)
-            z0 = _reduce("diag", torch.float64, self.diag, row)
-        return norm, z0
+        row = self.states[i:i + 1]
+        return _reduce("norm", torch.float64, row), _reduce("diag", torch.float64, self.diag, row)
 
 
 class Server:
-    """Continuous-batching executor on one GPU (see the module docstring)."""
+    """Continuous-batching executor on one GPU (see the module docstring).
+
+    max_active limits the requests on the GPU at once (None: slots and the
+    HBM budget decide); max_slots=1, max_active=1 runs one request at a time
+    on the same fused programs."""
 
     def __init__(self, hbm_budget: int = 64 << 30, max_slots: int = 32,
                  max_layers_per_step: int = 8, device="cuda", use_graphs: bool = True,
-                 graph_max_qubits: int = 20):
+                 max_active: Optional[int] = None):
         self.budget = hbm_budget
         self.max_slots = max_slots
         self.kmax = max_layers_per_step
@@ -224,10 +274,10 @@ class Server:
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         self.use_graphs = use_graphs
-        self.graph_max_qubits = graph_max_qubits
+        self.max_active = max_active
         self.lanes: Dict[int, _Lane] = {}
         self.used_bytes = 0
-        self.log: List[str] = []
+        self.active = 0
 
     def _lane_capacity(self, n: int) -> int:
         fp = (2 ** n) * BYTES_C64
@@ -236,6 +286,8 @@ class Server:
     def try_admit(self, r: Request, now: float) -> bool:
         if r.footprint > self.budget:
             raise ValueError(f"request {r.rid}: {r.footprint} B exceeds the HBM budget")
+        if self.max_active is not None and self.active >= self.max_active:
+            return False
         lane = self.lanes.get(r.n)
         if lane is None:
             cap = self._lane_capacity(r.n)
@@ -246,7 +298,7 @@ class Server:
                     return False
                 need = cap * r.footprint
             with on_device(self.device):
-                lane = _Lane(r.n, int(cap), self.kmax, self.device)
+                lane = _Lane(r.n, int(cap), self.kmax, self.device, self.use_graphs)
             self.lanes[r.n] = lane
             self.used_bytes += need
         slot = lane.free_slot()
@@ -259,6 +311,7 @@ class Server:
             lane.states[slot].zero_()
             lane.states[slot, 0] = 1
         r.admitted = now
+        self.active += 1
         return True
 
     def run(self, requests: List[Request], realtime: bool = True) -> List[Request]:
@@ -266,11 +319,13 @@ class Server:
         with results and completion times filled in."""
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting: List[Request] = []
-        done: List[Request] = []
-        t0 = time.perf_counter()
+        finished = 0
         i = 0
         with on_device(self.device):
-            while len(done) < len(requests):
+            start = torch.cuda.Event(enable_timing=True)
+            start.record()
+            t0 = time.perf_counter()
+            while finished < len(requests):
                 now = time.perf_counter() - t0 if realtime else float("inf")
                 while i < len(pending) and pending[i].arrival <= now:
                     waiting.append(pending[i])
@@ -280,31 +335,22 @@ class Server:
                     if not self.try_admit(r, now):
                         still.append(r)
                 waiting = still
-                busy = [l for l in self.lanes.values() if l.active()]
-                if not busy:
-                    if i < len(pending) and realtime:
-                        time.sleep(max(0.0, min(0.001, pending[i].arrival - now)))
-                    elif waiting:
-                        raise RuntimeError("requests cannot be admitted under the budget")
-                    continue
-                for lane in busy:
-                    lane.step(self.use_graphs and lane.n <= self.graph_max_qubits)
-                results = []
-                for lane in busy:
-                    for s in lane.finished():
-                        results.append((lane, s, lane.slots[s]) + lane.collect(s))
-                torch.cuda.synchronize(self.device)
-                tnow = time.perf_counter() - t0
-                for lane, s, r, norm, z0 in results:
-                    r.norm2 = float(norm)
-                    r.z0 = float(z0)
-                    r.done = tnow
-                    lane.slots[s] = None
-                    del lane.data[r.rid]
-                    done.append(r)
-                for n in [n for n, l in self.lanes.items() if not l.active()]:
+                progress = False
+                for lane in list(self.lanes.values()):
+                    got = lane.harvest(start)
+                    finished += len(got)
+                    self.active -= len(got)
+                    progress = progress or bool(got)
+                    while lane.active() and len(lane.inflight) < _Lane.NBUF:
+                        lane.issue()
+                        progress = True
+                for n in [n for n, l in self.lanes.items() if l.idle()]:
                     self.used_bytes -= self.lanes[n].bytes
                     del self.lanes[n]
+                if not progress:
+                    if waiting and not self.lanes and i >= len(pending):
+                        raise RuntimeError("requests cannot be admitted under the budget")
+                    time.sleep(2e-5)
         return requests
 
 
@@ -330,6 +376,13 @@ def run_one_at_a_time(requests: List[Request], device="cuda", realtime: bool = T
         r.done = time.perf_counter() - t0
         r.norm2, r.z0 = float(norm[0]), float(z0[0])
     return requests
+
+
+def run_one_at_a_time_fused(requests: List[Request], device="cuda", realtime: bool = True):
+    """One request at a time on the server's own fused programs (one slot,
+    one request on the GPU): the server's gain over this is batching and
+    overlap alone, not fusion."""
+    return Server(max_slots=1, max_active=1, device=device).run(requests, realtime=realtime)
 
 
 def summarize(requests: List[Request]):

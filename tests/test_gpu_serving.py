@@ -1,11 +1,13 @@
-"""Serving loop: results against the complex128 oracle, the same result
-alone and in the mix, admission under a small HBM budget, and Poisson
-arrivals end to end."""
+"""Serving loop: results against the complex128 oracle (small and 21-22
+qubit lanes), the same result alone and in the mix, admission under a small
+HBM budget and while lanes run, and Poisson arrivals end to end against both
+one-at-a-time baselines."""
 import numpy as np
 import pytest
 import torch
 
-from unitair_amd.serving import Request, Server, poisson_workload, run_one_at_a_time
+from unitair_amd.serving import (Request, Server, poisson_workload, run_one_at_a_time,
+                                 run_one_at_a_time_fused)
 from oracle import circuits as oc
 
 pytestmark = pytest.mark.gpu
@@ -67,3 +69,32 @@ def test_poisson_arrivals_and_one_at_a_time(device):
     for a, b in zip(reqs, base):
         assert a.done >= a.arrival and b.done >= b.arrival
         assert abs(a.z0 - b.z0) < 1e-5 and abs(a.norm2 - b.norm2) < 1e-5
+
+
+def test_large_lanes_match_oracle(device):
+    """21- and 22-qubit lanes (two slots each, several steps, graph replay)
+    against the complex128 oracle."""
+    reqs = [Request(i, n, d, seed=300 + i) for i, (n, d) in
+            enumerate([(21, 6), (22, 5), (21, 9), (12, 20)])]
+    Server(device=device, max_layers_per_step=4).run(reqs, realtime=False)
+    for r in reqs:
+        norm, z0 = oracle_answer(r)
+        assert abs(r.norm2 - 1.0) < 1e-5
+        assert abs(r.z0 - z0) < 1e-5, (r.n, r.depth)
+
+
+def test_admission_while_lanes_run_and_fused_baseline(device):
+    """Requests arriving while a lane's steps are in flight join freed or
+    free slots (at most two steps in flight, no device-wide waits); each
+    result equals the one-at-a-time run on the same fused programs within
+    the complex64 tolerance (the two runs cut the layers into different
+    steps, so stage grouping, hence rounding, may differ)."""
+    reqs = poisson_workload(16, rate=400.0, seed=8, n_range=(10, 15), depth_range=(10, 60))
+    server = Server(device=device, max_slots=4, max_layers_per_step=3)
+    server.run(reqs, realtime=True)
+    alone = poisson_workload(16, rate=400.0, seed=8, n_range=(10, 15), depth_range=(10, 60))
+    run_one_at_a_time_fused(alone, device=device, realtime=True)
+    for a, b in zip(reqs, alone):
+        assert a.done >= a.arrival and b.done >= b.arrival
+        assert abs(a.norm2 - b.norm2) < 1e-6 and abs(a.z0 - b.z0) < 1e-6
+    assert not server.lanes and server.used_bytes == 0 and server.active == 0
