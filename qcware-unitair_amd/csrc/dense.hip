@@ -154,6 +154,13 @@ bool aligned(const void* ptr, int64_t stride_elems, size_t esize, size_t align) 
 
 }  // namespace
 
+// Tiled dense gates on 6..kMaxWideGateK qubits (wide.hip).
+constexpr int kMaxWideGateK = 10;
+bool wide_applies(int n, int k);
+int apply_dense_wide(void* out, const void* in, const void* op, int dtype, int n, int k,
+                     const int* tb, int64_t batch, int64_t in_bstride, int64_t out_bstride,
+                     int64_t op_bstride, hipStream_t s);
+
 // Validated entry used by ua_apply_operator and by other host code.
 int apply_operator_impl(void* out, const void* state, const void* op, int dtype,
                         int n, int k, const int* qubits, int64_t batch,
@@ -247,8 +254,15 @@ int apply_operator_impl(void* out, const void* state, const void* op, int dtype,
     return check_launch("dense_fast_kernel");
   }
 
-  UA_CHECK_ARG(!in_place, "in-place application supports operators on 1..%d qubits",
-               kMaxFastK);
+  if (wide_applies(n, k)) {  // 6..10 targets: tiled, in place or not
+    int tbc[kMaxWideGateK];
+    for (int m = 0; m < k; ++m) tbc[m] = n - 1 - qubits[m];
+    return apply_dense_wide(out, state, op, dtype, n, k, tbc, batch, state_bstride, N,
+                            op_bstride, s);
+  }
+  UA_CHECK_ARG(!in_place,
+               "in-place application supports operators on 1..%d qubits (6..%d with at "
+               "least 12 qubits)", kMaxFastK, kMaxWideGateK);
   UA_CHECK_ARG(k <= 20, "operators on more than 20 qubits are not supported");
   GenericParams g{};
   g.total = batch * N;
