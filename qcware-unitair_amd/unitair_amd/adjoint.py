@@ -106,17 +106,38 @@ def _inverse_fixed(it: _Item) -> _Item:
     return _Item(it.kind, it.bits, np.conj(it.data).T.copy())
 
 
-def _rot_items(rot, params, dtype, n, sign, repeat=1):
-    """Fused-pass items of a rotation group, angles sign * coef * params."""
-    items = []
-    for kind, q, idx, coef in rot:
-        t = (sign * coef) * params[:, idx]
-        data = _rotation_data(kind, t, dtype)
+def _rotation_table(spec, params, dtype, sign, repeat=1):
+    """Per-state data of every rotation of the circuit, angles
+    sign * coef * params[:, idx], made with a few batched ops per Pauli kind:
+    {(segment, position in its group): (B * repeat, 2, 2) or (B * repeat, 2)
+    contiguous tensor}."""
+    table = {}
+    for kind in _PAULI:
+        keys, idxs, coefs = [], [], []
+        for s, (_, rot) in enumerate(spec.segments):
+            for r, (k, q, idx, coef) in enumerate(rot):
+                if k == kind:
+                    keys.append((s, r))
+                    idxs.append(idx)
+                    coefs.append(sign * coef)
+        if not keys:
+            continue
+        dev = params.device
+        t = (params[:, torch.tensor(idxs, device=dev)] *
+             torch.tensor(coefs, dtype=params.dtype, device=dev)).t()      # (R, B)
+        data = _rotation_data(kind, t, dtype)                               # (R, B, ...)
         if repeat > 1:
-            data = data.repeat((repeat,) + (1,) * (data.dim() - 1))
+            data = data.repeat((1, repeat) + (1,) * (data.dim() - 2))
         data = data.contiguous()
-        items.append(_Item("diag1" if kind == "z" else "dense1", (n - 1 - q,), data, True))
-    return items
+        for i, key in enumerate(keys):
+            table[key] = data[i]
+    return table
+
+
+def _rot_items(s, rot, table, n):
+    """Fused-pass items of rotation group s (its per-state data from table)."""
+    return [_Item("diag1" if kind == "z" else "dense1", (n - 1 - q,), table[(s, r)], True)
+            for r, (kind, q, idx, coef) in enumerate(rot)]
 
 
 def _program(items, n, dtype, device):
@@ -147,20 +168,23 @@ class _Programs:
         n = spec.n
         S = len(spec.segments)
         self.every = max(1, math.ceil(math.sqrt(max(S, 1))))
+        fwd_table = _rotation_table(spec, p, dtype, +1.0)
         self.fwd = []
         for c0 in range(0, S, self.every):
             items = []
-            for fixed, rot in spec.segments[c0:c0 + self.every]:
-                items += list(fixed) + _rot_items(rot, p, dtype, n, +1.0)
+            for s in range(c0, min(S, c0 + self.every)):
+                fixed, rot = spec.segments[s]
+                items += list(fixed) + _rot_items(s, rot, fwd_table, n)
             self.fwd.append((c0, _program(items, n, dtype, dev)))
         self.trail = _program(list(spec.trailing), n, dtype, dev)
         self.inv_trail = self.inv = None
         if backward:
+            inv_table = _rotation_table(spec, p, dtype, -1.0, repeat=2)
             self.inv_trail = _program([_inverse_fixed(it) for it in reversed(spec.trailing)],
                                       n, dtype, dev)
             self.inv = []
-            for fixed, rot in spec.segments:
-                items = _rot_items(rot, p, dtype, n, -1.0, repeat=2)
+            for s, (fixed, rot) in enumerate(spec.segments):
+                items = _rot_items(s, rot, inv_table, n)
                 items += [_inverse_fixed(it) for it in reversed(fixed)]
                 self.inv.append(_program(items, n, dtype, dev))
 
@@ -284,9 +308,10 @@ def run_gates(params: torch.Tensor, gates: Sequence[tuple], num_qubits: int,
     B, dev = params.shape[0], params.device
     p = params.detach().to(_REAL[dtype])
     with on_device(dev):
+        table = _rotation_table(spec, p, dtype, +1.0)
         items = []
-        for fixed, rot in spec.segments:
-            items += list(fixed) + _rot_items(rot, p, dtype, n, +1.0)
+        for s, (fixed, rot) in enumerate(spec.segments):
+            items += list(fixed) + _rot_items(s, rot, table, n)
         items += list(spec.trailing)
         prog = _program(items, n, dtype, dev)
         psi = _initial(B, n, dtype, dev, initial_state)

@@ -191,10 +191,70 @@ def _commute(a: _Item, b: _Item) -> bool:
     return False
 
 
-def _predecessors(items: Sequence[_Item]):
-    """preds[j] = the earlier items that item j does not commute with."""
-    return [{i for i in range(j) if not _commute(items[i], items[j])}
-            for j in range(len(items))]
+def _bit_actions(it: _Item, n: int):
+    """How the gate acts on each bit it touches: "Z" (diagonal there), "X"
+    (a CNOT target) or "G" (anything else).  Full-length phases and CZ layers
+    are diagonal on every bit."""
+    if it.kind in ("phase", "cz_layer"):
+        return [(b, "Z") for b in range(n)]
+    if it.kind in ("cz", "diag1"):
+        return [(b, "Z") for b in it.bits]
+    if it.kind == "cnot":
+        return [(it.bits[0], "Z"), (it.bits[1], "X")]
+    return [(b, "G") for b in it.bits]
+
+
+def _predecessors(items: Sequence[_Item], n: int):
+    """preds[j]: earlier items that item j must follow.  On one bit, gates of
+    the same action type Z or X commute (the rule of _commute), so a run of
+    them forms a group; a gate depends, on each of its bits, on the last
+    group it does not join (and a wide gate on everything before it).  Every
+    earlier gate that does not commute with j is an ancestor of preds[j]."""
+    groups = {}                      # bit -> (type, members, previous group)
+    preds = []
+    for j, it in enumerate(items):
+        if it.kind == "wide":
+            preds.append(set(range(j)))
+            for b in range(n):
+                groups[b] = ("G", [j], [])
+            continue
+        deps = set()
+        for b, t in _bit_actions(it, n):
+            g = groups.get(b)
+            if g is None:
+                groups[b] = (t, [j], [])
+                continue
+            gt, members, prev = g
+            if t == gt and t in ("Z", "X"):
+                deps.update(prev)
+                members.append(j)
+            else:
+                deps.update(members)
+                groups[b] = (t, [j], members)
+        preds.append(deps)
+    return preds
+
+
+class _Ready:
+    """Items whose predecessors are all placed, kept up to date by counters."""
+
+    def __init__(self, preds):
+        self.succ = [[] for _ in preds]
+        for j, ps in enumerate(preds):
+            for i in ps:
+                self.succ[i].append(j)
+        self.waiting = [len(ps) for ps in preds]
+        self.ready = [j for j in range(len(preds)) if self.waiting[j] == 0]
+
+    def place(self, j):
+        self.ready.remove(j)
+        for k in self.succ[j]:
+            self.waiting[k] -= 1
+            if self.waiting[k] == 0:
+                self.ready.append(k)
+
+    def sorted(self):
+        return sorted(self.ready)
 
 
 def _plan_passes_in_order(items: Sequence[_Item], n: int, tile_bits: int = TILE_BITS):
@@ -253,26 +313,21 @@ def plan_passes(items: Sequence[_Item], n: int, tile_bits: int = TILE_BITS,
     m = min(n, tile_bits)
     base = set(range(min(n, CUBE_BITS)))
     items = list(items)
-    preds = _predecessors(items)
-    done = [False] * len(items)
-    left = len(items)
+    rd = _Ready(_predecessors(items, n))
     out = []
-    while left:
-        ready = [j for j in range(len(items)) if not done[j] and all(done[i] for i in preds[j])]
-        wide = [j for j in ready if items[j].kind == "wide"]
-        narrow = [j for j in ready if items[j].kind != "wide"]
-        if not narrow:
-            j = wide[0]
-            done[j] = True
-            left -= 1
+    while rd.ready:
+        order = rd.sorted()
+        if all(items[j].kind == "wide" for j in order):
+            j = order[0]
+            rd.place(j)
             out.append(items[j])
             continue
         local = set(base)
         taken = []
         while True:
             best = None
-            for j in range(len(items)):
-                if done[j] or items[j].kind == "wide" or not all(done[i] for i in preds[j]):
+            for j in rd.sorted():
+                if items[j].kind == "wide":
                     continue
                 grow = len(_needs(items[j]) - local)
                 if len(local) + grow > m:
@@ -285,8 +340,7 @@ def plan_passes(items: Sequence[_Item], n: int, tile_bits: int = TILE_BITS,
                 break
             j = best[1]
             local |= _needs(items[j])
-            done[j] = True
-            left -= 1
+            rd.place(j)
             taken.append(items[j])
         out.append(_Pass(local=[], items=taken))
     # pad each pass's local set to m bits and fix its order
@@ -367,22 +421,17 @@ def _stages(p: _Pass, reorder: bool = True):
     tile_of = {s: i for i, s in enumerate(p.local)}
     m = len(p.local)
     items = list(p.items)
-    preds = _predecessors(items)
-    done = [False] * len(items)
-    left = len(items)
-
-    def ready(j):
-        return not done[j] and all(done[i] for i in preds[j])
-
+    nbits = 1 + max([b for it in items if it.kind not in ("phase", "cz_layer")
+                     for b in it.bits] + list(p.local) + [0])
+    rd = _Ready(_predecessors(items, nbits))
     stages = []
-    while left:
-        cubeable = [j for j in range(len(items)) if ready(j) and items[j].kind != "phase"]
-        if cubeable:
+    while rd.ready:
+        if any(items[j].kind != "phase" for j in rd.ready):
             cube, ops = set(), []
             while True:
                 best = None
-                for j in range(len(items)):
-                    if not ready(j) or items[j].kind == "phase":
+                for j in rd.sorted():
+                    if items[j].kind == "phase":
                         continue
                     grow = len({tile_of[b] for b in _needs(items[j])} - cube)
                     if len(cube) + grow > CUBE_BITS:
@@ -395,19 +444,17 @@ def _stages(p: _Pass, reorder: bool = True):
                     break
                 j = best[1]
                 cube |= {tile_of[b] for b in _needs(items[j])}
-                done[j] = True
-                left -= 1
+                rd.place(j)
                 ops.append(items[j])
             stages.append((cube, ops))
             continue
         ops = []
         while True:
-            diag = [j for j in range(len(items)) if ready(j) and items[j].kind in _DIAG]
+            diag = [j for j in rd.sorted() if items[j].kind in _DIAG]
             if not diag:
                 break
             for j in diag:
-                done[j] = True
-                left -= 1
+                rd.place(j)
                 ops.append(items[j])
         stages.append((None, ops))
     result = []
