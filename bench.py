@@ -447,7 +447,8 @@ def _ops_inputs(n, seed, device, dtype):
     rng = np.random.default_rng(seed)
     psi = seeded_state(n, (), seed=seed, dtype=dtype, device=device)
     other = seeded_state(n, (), seed=seed + 1, dtype=dtype, device=device)
-    u = {k: torch.from_numpy(haar_unitary(2 ** k, rng)).to(device, dtype) for k in (1, 2, 5)}
+    u = {k: torch.from_numpy(haar_unitary(2 ** k, rng)).to(device, dtype)
+         for k in (1, 2, 5, 6, 8)}
     ang = torch.from_numpy(rng.uniform(0, 2 * np.pi, 2 ** n).astype(np.float32)).to(device)
     diag = torch.from_numpy(rng.standard_normal(2 ** n).astype(np.float32)).to(device)
     return psi, other, u, ang, diag
@@ -479,6 +480,10 @@ def _op_table(n, sim, states, psi, other, u, ang, diag, sample, rand_state):
          lambda: sim.apply_operator(u[2], [5, 17], psi), 2 * A * s),
         ("apply_operator 5q Haar, qubits (0..4)",
          lambda: sim.apply_operator(u[5], [0, 1, 2, 3, 4], psi), 2 * A * s),
+        ("apply_operator 6q Haar, qubits (0, 7, 12, 3, 23, 18) (tiled wide-gate kernel)",
+         lambda: sim.apply_operator(u[6], [0, 7, 12, 3, 23, 18], psi), 2 * A * s),
+        ("apply_operator 8q Haar, qubits (2, 9, 14, 0, 21, 5, 17, 11) (tiled wide-gate kernel)",
+         lambda: sim.apply_operator(u[8], [2, 9, 14, 0, 21, 5, 17, 11], psi), 2 * A * s),
         ("act_first_qubits 2q Haar",
          lambda: sim.act_first_qubits(u[2], psi), 2 * A * s),
         ("apply_phase, full-length float32 angles",

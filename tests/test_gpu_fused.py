@@ -49,7 +49,9 @@ def test_fused_random_circuit_24q(device, dt):
     fid = float(U.fidelity_error(ref, out))
     mae = U.max_abs_error(ref, out)
     print(f"fused 24q {dt}: {prog.num_passes} passes, fidelity error {fid:.3e}, max-abs {mae:.3e}")
-    assert fid < TOL[dt] and mae < TOL[dt]
+    # achieved: complex64 ~3e-13 / ~6e-10, complex128 ~1e-16 / ~1e-17
+    bar = {torch.complex64: (1e-10, 1e-7), torch.complex128: (1e-14, 1e-14)}[dt]
+    assert fid < bar[0] and mae < bar[1], (fid, mae)
 
 
 def test_fused_graph_replay_matches_eager(device):

@@ -57,12 +57,50 @@ def test_mfma_equals_vector_path_closely(device, qubits):
         assert rel(a.cpu().numpy(), b.cpu().numpy()) < TOL[dt]
 
 
+# achieved on MI355X: complex64 fidelity error ~1e-12 (max-abs ~5e-9),
+# complex128 ~2e-16; the bars leave two to three orders of margin
+SLICE_BAR = {torch.complex64: (1e-9, 1e-6), torch.complex128: (1e-13, 1e-12)}
+
+
 @pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
 def test_fused_dense_circuit_20q_slice(device, dt):
-    circ = fused_dense_circuit(seed=4, n=20, n5=12, n4=4)
+    """The dense-block workload's full gate list (64 Haar 32x32 + 16 Haar
+    16x16 blocks) at 20 qubits, against the complex128 closed form."""
+    circ = fused_dense_circuit(seed=4, n=20)
+    assert circ.counts() == {"dense5": 64, "dense4": 16}
     psi = seeded_state_np(20, (), seed=8)
     ref = oc.run_closed_form(circ.gates, psi)
     out = CircuitRunner(circ, dtype=dt, device=device).run(
         torch.from_numpy(psi).to(device, dt).reshape(1, -1)).cpu().numpy()[0]
-    fid = float(U.fidelity_error(ref, out))
-    assert fid < TOL[dt] and U.max_abs_error(ref, out) < TOL[dt]
+    fid, mae = float(U.fidelity_error(ref, out)), U.max_abs_error(ref, out)
+    assert fid < SLICE_BAR[dt][0] and mae < SLICE_BAR[dt][1], (fid, mae)
+
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("qubits", [(0, 1, 2, 3, 4), (23, 24, 25, 26, 27), (27, 3, 14, 0, 22),
+                                    (2, 27, 9, 16)])
+def test_mfma_28q_unitary_roundtrip(device, dt, qubits):
+    """4- and 5-qubit Haar blocks on a 28-qubit state (the dense-block
+    configuration's size; the highest and lowest strides): the norm is kept
+    and U then U^H returns the input, in place through the matrix cores."""
+    from unitair_amd.gates import haar_unitary
+    from unitair_amd.initializations import rand_state
+    import unitair_amd as ua
+    rng = np.random.default_rng(sum(qubits))
+    u = torch.from_numpy(haar_unitary(2 ** len(qubits), rng)).to(device, dt)
+    uh = u.conj().T.contiguous()
+    psi = rand_state(28, device=device, dtype=dt, seed=5)
+    lib = _lib.load()
+    x = psi.clone()
+    q = _lib.int_array(list(qubits))
+    for m in (u, uh):
+        rc = lib.ua_apply_operator(_lib.ptr(x), _lib.ptr(x), _lib.ptr(m), _lib.DTYPE_CODES[dt],
+                                   28, len(qubits), q, 1, 2 ** 28, 0, _lib.stream_handle(device))
+        _lib.check(rc, "apply_operator")
+        if m is u:
+            assert abs(float(ua.states.norm_squared_f64(x)) - 1.0) < (1e-5 if dt == torch.complex64
+                                                                     else 1e-12)
+    err = float((x - psi).abs().max()) / float(psi.abs().max())
+    assert err < (1e-5 if dt == torch.complex64 else 1e-12), err
+    del x, psi
+    torch.cuda.empty_cache()

@@ -298,7 +298,9 @@ def _circuit_24q_and_oracle():
 @pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
 def test_random_circuit_24q(device, dt):
     """24 qubits, 200 gates: fidelity error and max-abs error vs the complex128
-    oracle (1e-5 for complex64, 1e-12 for complex128)."""
+    oracle.  The stated bars are 1e-5 (complex64) and 1e-12 (complex128);
+    achieved is about 3e-13 / 6e-10 (complex64) and 1e-16 / 1e-17
+    (complex128), so the test holds the kernels to much tighter bars."""
     from unitair_amd.circuits import CircuitRunner
     circ, psi, ref = _circuit_24q_and_oracle()
     r = CircuitRunner(circ, dtype=dt, device=device, in_place=True)
@@ -306,9 +308,9 @@ def test_random_circuit_24q(device, dt):
     out = r.run(x).cpu().numpy()
     fid = float(U.fidelity_error(ref, out))
     mae = U.max_abs_error(ref, out)
-    tol = TOL[dt]
     print(f"24q {dt}: fidelity error {fid:.3e}, max-abs error {mae:.3e}")
-    assert fid < tol and mae < tol
+    bar = {torch.complex64: (1e-10, 1e-7), torch.complex128: (1e-14, 1e-14)}[dt]
+    assert fid < bar[0] and mae < bar[1], (fid, mae)
 
 
 def test_norm_and_inverse_at_28q(device):

@@ -532,3 +532,31 @@ def test_general_rotation_oracle_pins():
     pm[i] -= h
     fd = (AO.general_loss(pp, gates, obs, n) - AO.general_loss(pm, gates, obs, n)) / (2 * h)
     assert abs(wrong(params) - fd) > 1e-4
+
+
+def test_fidelity_and_max_abs_error_pins():
+    """The acceptance metrics (oracle/unitair_np.py fidelity_error and
+    max_abs_error) by closed forms: identical states give 0; a global phase
+    and a positive scale give 0 fidelity error; orthogonal states give 1; a
+    state rotated by angle a against the reference gives sin(a)^2; batches
+    are per state; max-abs is the largest elementwise modulus of the
+    difference.  A planted mistake (no conjugate in the overlap) fails."""
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(64) + 1j * rng.standard_normal(64)
+    b = rng.standard_normal(64) + 1j * rng.standard_normal(64)
+    b -= np.vdot(a, b) / np.vdot(a, a) * a          # orthogonal to a
+    assert abs(U.fidelity_error(a, a)) < 1e-15
+    assert abs(U.fidelity_error(a, 2.5 * np.exp(0.7j) * a)) < 1e-15
+    assert abs(U.fidelity_error(a, b) - 1.0) < 1e-12
+    an, bn = a / np.linalg.norm(a), b / np.linalg.norm(b)
+    for ang in (0.1, 0.5, 1.3):
+        x = np.cos(ang) * an + np.sin(ang) * bn
+        assert abs(U.fidelity_error(an, x) - np.sin(ang) ** 2) < 1e-12
+    batch = U.fidelity_error(np.stack([a, a]), np.stack([a, b]))
+    assert batch.shape == (2,) and abs(batch[0]) < 1e-15 and abs(batch[1] - 1) < 1e-12
+    d = np.zeros(64, complex)
+    d[5] = 3 + 4j
+    assert U.max_abs_error(a, a + d) == 5.0
+    # planted: the overlap without the conjugate is not phase-invariant
+    wrong = 1 - abs(np.sum(a * (np.exp(0.7j) * a))) ** 2 / (np.vdot(a, a).real ** 2)
+    assert abs(wrong) > 1e-3
