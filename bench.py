@@ -385,32 +385,42 @@ def fused_dense_bench(dev, steps=3, warmup=1, seed=4, n=28):
     for dt, name in ((torch.complex64, "c64"), (torch.complex128, "c128")):
         state = base.to(dt) if dt != torch.complex128 else base
         es = state.element_size()
-        runner = CircuitRunner(circ, dtype=dt, device=dev, in_place=True)
-        sec, per_kind = time_steps(runner, state, steps, warmup, with_events=True)
-        step_bytes = len(circ.gates) * 2 * amps * es
         peak = MFMA_PEAK_TFLOPS[name]
-        kernels = {}
-        for kind, v in per_kind.items():
-            k = int(kind[len("dense"):-1])
-            us = 1e3 * float(np.mean(v))
-            kflops = 8 * 2 ** k * amps
-            kernels[kind] = {"launches_per_step": len(v) // steps, "avg_us": us,
-                             "algorithmic_GBps": 2 * amps * es / us / 1e3,
-                             "TFLOPs": kflops / us / 1e6,
-                             "mfma_frac": kflops / us / 1e6 / peak}
-        out[name] = {"ms_per_step": sec * 1e3,
-                     "amp_updates_per_s": len(circ.gates) * amps / sec,
-                     "algorithmic_GBps": step_bytes / sec / 1e9,
-                     "hbm_frac": step_bytes / sec / 1e9 / HBM_PEAK_GBS,
-                     "TFLOPs": flops / sec / 1e12,
-                     "mfma_frac": flops / sec / 1e12 / peak,
-                     "roofline_ms": 1e3 * max(step_bytes / (HBM_PEAK_GBS * 1e9),
-                                              flops / (peak * 1e12)),
-                     "kernels": kernels}
-        log(f"fused dense {name}: {sec * 1e3:.2f} ms/step, "
-            f"{out[name]['algorithmic_GBps']:.0f} GB/s, {out[name]['TFLOPs']:.1f} TFLOP/s; "
-            + ", ".join(f"{k}={v['avg_us']:.0f}us" for k, v in kernels.items()))
-        del runner, state
+
+        def leg(runner, label):
+            sec, per_kind = time_steps(runner, state, steps, warmup, with_events=True)
+            launches = len(runner.kinds)
+            step_bytes = launches * 2 * amps * es
+            kernels = {}
+            for kind, v in per_kind.items():
+                ks = [int(c) for c in kind[len("blocks"):]] if kind.startswith("blocks") \
+                    else [int(kind[len("dense"):-1])]
+                us = 1e3 * float(np.mean(v))
+                kflops = sum(8 * 2 ** k * amps for k in ks)
+                kernels[kind] = {"launches_per_step": len(v) // steps, "avg_us": us,
+                                 "algorithmic_GBps": 2 * amps * es / us / 1e3,
+                                 "TFLOPs": kflops / us / 1e6,
+                                 "mfma_frac": kflops / us / 1e6 / peak}
+            res = {"ms_per_step": sec * 1e3, "passes_over_state": launches,
+                   "amp_updates_per_s": len(circ.gates) * amps / sec,
+                   "algorithmic_GBps": step_bytes / sec / 1e9,
+                   "hbm_frac": step_bytes / sec / 1e9 / HBM_PEAK_GBS,
+                   "TFLOPs": flops / sec / 1e12,
+                   "mfma_frac": flops / sec / 1e12 / peak,
+                   "roofline_ms": 1e3 * max(step_bytes / (HBM_PEAK_GBS * 1e9),
+                                            flops / (peak * 1e12)),
+                   "kernels": kernels}
+            log(f"fused dense {name} {label}: {sec * 1e3:.2f} ms/step, {launches} passes, "
+                f"{res['algorithmic_GBps']:.0f} GB/s, {res['TFLOPs']:.1f} TFLOP/s")
+            return res
+
+        blocks = CircuitRunner(circ, dtype=dt, device=dev, in_place=True, dense_blocks=True)
+        res = leg(blocks, "block passes")
+        del blocks
+        per_gate = CircuitRunner(circ, dtype=dt, device=dev, in_place=True)
+        res["per_gate"] = leg(per_gate, "one gate per pass")
+        out[name] = res
+        del per_gate, state
         torch.cuda.empty_cache()
     del base
     torch.cuda.empty_cache()
@@ -431,10 +441,12 @@ def fused_dense_slice_fidelity(dev, seed=4, slice_qubits=20):
     ref = oc.run_closed_form(sl.gates, psi)
     out = {"slice": f"same builder at {slice_qubits} qubits ({len(sl.gates)} gates)"}
     for dt, name in ((torch.complex64, "c64"), (torch.complex128, "c128")):
-        r = CircuitRunner(sl, dtype=dt, device=dev, in_place=True)
-        got = r.run(torch.from_numpy(psi).to(dev, dt)).cpu().numpy()
-        out[name] = {"fidelity_error": float(U.fidelity_error(ref, got)),
-                     "max_abs_error": float(U.max_abs_error(ref, got))}
+        for label, blocks in (("block_passes", True), ("per_gate", False)):
+            r = CircuitRunner(sl, dtype=dt, device=dev, in_place=True, dense_blocks=blocks)
+            got = r.run(torch.from_numpy(psi).to(dev, dt)).cpu().numpy()
+            out.setdefault(name, {})[label] = {
+                "fidelity_error": float(U.fidelity_error(ref, got)),
+                "max_abs_error": float(U.max_abs_error(ref, got))}
     return out
 
 

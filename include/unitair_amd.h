@@ -316,6 +316,29 @@ int ua_run_circuit(void* state, void* work, int dtype, int num_qubits, int64_t b
                    int* result_in_work, ua_stream_t stream);
 
 /*
+ * Dense-block passes.  unitair applies every gate as its own contraction over
+ * the state (unitair/simulation/operations.py:151-186, 322; one
+ * apply_operator call per gate).  ua_run_dense_blocks applies a gate list in
+ * place like ua_run_circuit with work == NULL, but consecutive dense gates on
+ * 4 or 5 qubits (complex64 / complex128 states of at least 14 qubits) whose
+ * targets fit together in a 12-bit tile (their union plus index bits 0 and 1)
+ * share one pass over the state on the matrix cores: up to 3 gates per pass
+ * in complex64, 2 in complex128.  Every other gate is one launch, as in
+ * ua_run_circuit.  Results equal gate-by-gate application to rounding.
+ *
+ * ua_plan_dense_blocks -- the launch units of that schedule, without running
+ * it: *num_units units; unit_of_gate (NULL, or num_gates ints) receives each
+ * gate's unit.  Validates the gate list like ua_run_dense_blocks.
+ * ua_run_dense_blocks -- events: NULL, or 2 * num_units hipEvent_t handles
+ * recorded on `stream` before and after each unit.
+ */
+int ua_plan_dense_blocks(int dtype, int num_qubits, const ua_gate* gates, int num_gates,
+                         int* unit_of_gate, int* num_units);
+int ua_run_dense_blocks(void* state, int dtype, int num_qubits, int64_t batch,
+                        const ua_gate* gates, int num_gates, void* const* events,
+                        ua_stream_t stream);
+
+/*
  * Fused passes: several gates per read and write of the state.  The
  * reference's only fusion is multiplying 1-qubit gates that hit the same qubit
  * (unitair/gates/matrix_algebra.py:6-51, apply_to_qubits operations.py:

@@ -1,33 +1,24 @@
-// Dense 4- and 5-qubit gates on the matrix cores (MFMA).
+// Dense 4- and 5-qubit gates on the matrix cores (MFMA), one gate per launch.
 //
 // For a k-qubit gate (D = 2^k) every group of D amplitudes that differ only
 // in the target bits is multiplied by U (operations.py:322 contracts the
-// same way).  Written as real arithmetic, the gate is the 2D x 2D block
-//     [ Re U  -Im U ]   acting on the columns   [ Re x ]
-//     [ Im U   Re U ]                            [ Im x ]
-// so a batch of 64 groups is a (2D x 2D) x (2D x 64) real GEMM: 8 D^2 flops
-// per group, for 16 D bytes moved (complex64).  At D = 32 that is 16 flop/B,
-// more than the vector units sustain without packed FP32, so these gates go
-// to v_mfma_f32_16x16x4f32 (complex64) and v_mfma_f64_16x16x4f64
-// (complex128).
+// same way).  In real arithmetic a batch of 64 groups is a (2D x 2D) x
+// (2D x 64) GEMM (mfma_common.h): 8 D^2 flops per group, for 16 D bytes moved
+// (complex64).  At D = 32 that is 16 flop/B, more than the vector units
+// sustain without packed FP32, so these gates go to the matrix cores.
 //
 // Workgroup = 4 wavefronts; per chunk it stages 64 groups in LDS as real and
 // imaginary rows, each wavefront multiplies 16 of them, and the results go
 // back through LDS so that global reads and writes stay coalesced.  The U
 // block lives in registers as MFMA A-operands for the whole (persistent)
-// launch.  Lane layouts of the two instructions were measured on gfx950:
-//   A: lane l holds A[l%16][l/16]; B: lane l holds B[l/16][l%16];
-//   result register r: f32 -> row 4*(l/16)+r, f64 -> row l/16+4r; col l%16.
-#include "ua_common.h"
+// launch.
+#include "mfma_common.h"
 
 namespace ua {
 
 bool g_use_mfma = true;
 
 namespace {
-
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 constexpr int kGroupsPerChunk = 64;
 constexpr int kMaxK = 5;
@@ -39,26 +30,6 @@ struct MfmaParams {
   int tb[kMaxK];         // ascending target bits
   int gpos[kMaxK];       // gate-index bit of canonical bit p
   int low_targets;       // lowest target bit < 6: stage groups with j fastest
-};
-
-template <typename T> struct RealOf;
-template <> struct RealOf<float2> { using type = float; };
-template <> struct RealOf<double2> { using type = double; };
-
-template <typename R> struct Mfma;
-template <> struct Mfma<float> {
-  using acc_t = floatx4;
-  __device__ static acc_t mma(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-template <> struct Mfma<double> {
-  using acc_t = doublex4;
-  __device__ static acc_t mma(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
 template <int K>
@@ -113,8 +84,7 @@ __global__ __launch_bounds__(kBlock) void dense_mfma_kernel(T* out, const T* in,
       const int row = 16 * ti + (lane & 15), col = 4 * ks + (lane >> 4);
       const int rr = row & (D - 1), cc = col & (D - 1);
       const T u = op[orig_index<K>(rr, p) * D + orig_index<K>(cc, p)];
-      const bool im_row = row >= D, im_col = col >= D;
-      a[ti][ks] = (!im_row && !im_col) ? u.x : (!im_row ? -u.y : (!im_col ? u.y : u.x));
+      a[ti][ks] = real_block(u, row >= D, col >= D);
     }
 
   const int64_t chunks = p.total_groups / kGroupsPerChunk;  // whole chunks (host-checked)

@@ -124,6 +124,11 @@ SIGNATURES["ua_run_fused_pass"] = (_i32, [_p, _i32, _i32, _i64, ctypes.POINTER(F
 SIGNATURES["ua_run_circuit"] = (_i32, [_p, _p, _i32, _i32, _i64, ctypes.POINTER(Gate), _i32,
                                        ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.POINTER(ctypes.c_int), _p])
+SIGNATURES["ua_plan_dense_blocks"] = (_i32, [_i32, _i32, ctypes.POINTER(Gate), _i32,
+                                             ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int)])
+SIGNATURES["ua_run_dense_blocks"] = (_i32, [_p, _i32, _i32, _i64, ctypes.POINTER(Gate), _i32,
+                                            ctypes.POINTER(ctypes.c_void_p), _p])
 
 _lib = None
 

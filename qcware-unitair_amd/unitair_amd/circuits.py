@@ -80,10 +80,15 @@ class CircuitRunner:
     The gate list is converted once into the C ABI's ua_gate array (device
     pointers to the uploaded matrices/angles); run() is then a single call of
     ua_run_circuit, which launches every gate from C on torch's current
-    stream.  capture() records that call into a CUDA/HIP graph for replay."""
+    stream.  capture() records that call into a CUDA/HIP graph for replay.
+
+    dense_blocks=True (in place only) runs the list through
+    ua_run_dense_blocks instead: consecutive 4/5-qubit dense gates that fit
+    one 12-bit tile share a matrix-core pass over the state.  `kinds` then
+    names the launch units ("blocks" + the gates' sizes for a pass)."""
 
     def __init__(self, circuit: Circuit, dtype: torch.dtype = torch.complex64,
-                 device="cuda", in_place: bool = True):
+                 device="cuda", in_place: bool = True, dense_blocks: bool = False):
         self.lib = _lib.load()
         self.circuit = circuit
         self.n = circuit.num_qubits
@@ -133,6 +138,27 @@ class CircuitRunner:
         self.kinds = kinds
         self._work = None
         self._graph = None
+        self.dense_blocks = dense_blocks
+        if dense_blocks:
+            if not in_place:
+                raise ValueError("dense_blocks runs in place")
+            ng = len(circuit.gates)
+            unit_of = (ctypes.c_int * max(1, ng))()
+            nu = ctypes.c_int(0)
+            _lib.check(self.lib.ua_plan_dense_blocks(self.code, self.n, gates, ng, unit_of,
+                                                     ctypes.byref(nu)), "ua_plan_dense_blocks")
+            units = [[] for _ in range(nu.value)]
+            for i in range(ng):
+                units[unit_of[i]].append(i)
+            blocked = dtype.is_complex and self.n >= 14
+
+            def unit_kind(u):
+                g = circuit.gates[u[0]]
+                if blocked and g.kind == "dense" and len(g.qubits) in (4, 5):
+                    return "blocks" + "".join(str(len(circuit.gates[i].qubits)) for i in u)
+                return kinds[u[0]]
+            self.kinds = [unit_kind(u) for u in units]
+            self.units = units
 
     def _check_state(self, state: torch.Tensor):
         if state.device != self.device or state.dtype != self.dtype:
@@ -149,6 +175,14 @@ class CircuitRunner:
         bracketed by its two events on the current stream."""
         self._check_state(state)
         batch = state.numel() // (2 ** self.n)
+        if self.dense_blocks:
+            ev = None if events is None else ctypes.cast(events.handles,
+                                                         ctypes.POINTER(ctypes.c_void_p))
+            rc = self.lib.ua_run_dense_blocks(_lib.ptr(state), self.code, self.n, batch,
+                                              self._gates, len(self.circuit.gates), ev,
+                                              _lib.stream_handle(self.device))
+            _lib.check(rc, "ua_run_dense_blocks")
+            return state
         if self.in_place:
             src, work = state, None
         else:
