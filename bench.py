@@ -292,11 +292,15 @@ def adjoint_bench(dev, iters=3):
     plain_step(th[:Bp], ph[:Bp])
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
+    moved = adjoint.last_traffic["forward_bytes"] + adjoint.last_traffic["backward_bytes"]
     return {"workload": f"{n} qubits, {L} layers (RZ.RY on every qubit + CZ ring), loss "
                         f"sum_i <Z_i>, complex64, forward + gradient of all {2 * L * n} "
                         f"angles per state",
             "adjoint_batch": B, "adjoint_ms": adj_s * 1e3,
             "adjoint_states_per_s": B / adj_s,
+            "adjoint_algorithmic_bytes": moved,
+            "adjoint_algorithmic_GBps": moved / adj_s / 1e9,
+            "adjoint_frac_of_hbm_peak": moved / adj_s / 1e9 / HBM_PEAK_GBS,
             "plain_per_gate_autograd_batch": Bp, "plain_per_gate_autograd_ms": plain_s * 1e3,
             "plain_per_gate_autograd_states_per_s": Bp / plain_s,
             "speedup_states_per_s": (B / adj_s) / (Bp / plain_s)}

@@ -189,6 +189,20 @@ class _Programs:
                 self.inv.append(_program(items, n, dtype, dev))
 
 
+# Algorithmic HBM bytes of the last forward and backward (state passes, copies,
+# diagonal scaling and gradient reductions), for throughput reports.
+last_traffic = {"forward_bytes": 0, "backward_bytes": 0}
+
+
+def _bytes(x: torch.Tensor) -> int:
+    return x.numel() * x.element_size()
+
+
+def _run_counted(prog, x, key):
+    prog.run(x)
+    last_traffic[key] += len(prog.steps) * 2 * _bytes(x)
+
+
 def _initial(B, n, dtype, dev, initial):
     psi = torch.zeros(B, 2 ** n, dtype=dtype, device=dev)
     if initial is None:
@@ -211,15 +225,18 @@ class _AdjointExpectation(torch.autograd.Function):
         with on_device(dev):
             progs = _Programs(spec, p, dtype, dev, backward=True)
             psi = _initial(B, n, dtype, dev, initial)
+            last_traffic["forward_bytes"] = _bytes(psi)
             ckpt = {}
             for c0, prog in progs.fwd:
                 if c0 > 0:
                     ckpt[c0] = psi.clone()          # state before segment c0
+                    last_traffic["forward_bytes"] += 2 * _bytes(psi)
                 if prog is not None:
-                    prog.run(psi)
+                    _run_counted(prog, psi, "forward_bytes")
             if progs.trail is not None:
-                progs.trail.run(psi)
+                _run_counted(progs.trail, psi, "forward_bytes")
             loss = _diag_loss(obs, psi)
+            last_traffic["forward_bytes"] += _bytes(psi) + _bytes(obs)
         ctx.save_for_backward(params, observable)
         ctx.state = (spec, dtype, obs, psi, ckpt, progs)
         return loss
@@ -241,6 +258,8 @@ class _AdjointExpectation(torch.autograd.Function):
                 return grad_p, grad_obs, None, None, None
             X = torch.empty(2 * B, 2 ** n, dtype=dtype, device=dev)
             X[:B].copy_(psi)
+            # copy + diagonal scaling (read psi and obs, write lambda)
+            last_traffic["backward_bytes"] = 4 * _bytes(psi) + _bytes(obs)
             code = _lib.DTYPE_CODES
             rc = lib.ua_apply_diagonal(_lib.ptr(X[B:]), _lib.ptr(psi), code[dtype],
                                        _lib.ptr(obs), code[obs.dtype],
@@ -249,7 +268,7 @@ class _AdjointExpectation(torch.autograd.Function):
             _lib.check(rc, "apply_diagonal")
             del psi
             if progs.inv_trail is not None:
-                progs.inv_trail.run(X)
+                _run_counted(progs.inv_trail, X, "backward_bytes")
             ws = torch.empty(lib.ua_pauli_gradients_workspace(n, B), dtype=torch.uint8,
                              device=dev)
             vals = torch.empty(len(_PAULI), B, n, dtype=torch.float64, device=dev)
@@ -262,6 +281,7 @@ class _AdjointExpectation(torch.autograd.Function):
                                                 code[dtype], n, B, _PAULI[kind], _lib.ptr(ws),
                                                 _lib.stream_handle(dev))
                     _lib.check(rc, "pauli_gradients")
+                    last_traffic["backward_bytes"] += _bytes(X)
                 # one gather-add per group: grad[:, idx] += 2 coef vals[kind, :, q]
                 kinds = torch.tensor([_PAULI[r[0]] for r in rot], device=dev)
                 qs = torch.tensor([r[1] for r in rot], device=dev)
@@ -269,9 +289,10 @@ class _AdjointExpectation(torch.autograd.Function):
                 coef = torch.tensor([2.0 * r[3] for r in rot], dtype=torch.float64, device=dev)
                 grad.index_add_(1, idx, vals[kinds, :, qs].transpose(0, 1) * coef)
                 if progs.inv[s] is not None:
-                    progs.inv[s].run(X)
+                    _run_counted(progs.inv[s], X, "backward_bytes")
                 if s in ckpt:
                     X[:B].copy_(ckpt[s])         # psi before segment s, as the forward had it
+                    last_traffic["backward_bytes"] += 2 * _bytes(ckpt[s])
             grad_p = (grad * gloss.to(torch.float64).view(B, 1)).to(params.dtype)
         return grad_p, grad_obs, None, None, None
 

@@ -182,13 +182,14 @@ def test_adjoint_16q_batch1024_c64(device):
     l64, gt64, gp64 = grads[torch.complex64]
     l128, gt128, gp128 = grads[torch.complex128]
     assert np.max(np.abs(l64 - l128)) < 1e-4 * n
-    assert np.max(np.abs(gt64 - gt128)) < 1e-3 and np.max(np.abs(gp64 - gp128)) < 1e-3
+    # complex64: gradients of sum_i <Z_i> (|grad| <= 16) within 2e-4 of complex128
+    assert np.max(np.abs(gt64 - gt128)) < 2e-4 and np.max(np.abs(gp64 - gp128)) < 2e-4
     for b in (0, 777):
         assert abs(l128[b] - AO.loss(th[b], ph[b], n)) < 1e-10
         for (l, q) in ((0, 0), (4, 7), (9, 15)):
             ps = AO.param_shift(th[b], ph[b], n, "theta", l, q)
             assert abs(gt128[b, l, q] - ps) < 1e-9
-            assert abs(gt64[b, l, q] - ps) < 1e-3
+            assert abs(gt64[b, l, q] - ps) < 2e-4
             h = 1e-3
             tp, tm = th[b].copy(), th[b].copy()
             tp[l, q] += h
