@@ -280,6 +280,33 @@ int ua_apply_cnot(void* state, int dtype, int num_qubits, int control, int targe
                   int64_t batch, ua_stream_t stream);
 
 /*
+ * ua_fuse_1q_ops -- products of 1-qubit gates per qubit
+ * (fuse_single_qubit_operators, unitair/gates/matrix_algebra.py:6-51):
+ * out[b, s] = U_m ... U_2 U_1 over the operators i with slot[i] == s, in list
+ * order (later operators on the left), for b < batch.  ops: num_ops device
+ * pointers (a host array), each to row-major 2x2 matrices, op_batch_stride
+ * elements apart per batch entry (0: one matrix for every entry).  out:
+ * (batch, num_slots, 2, 2) contiguous -- the operand layout of ua_apply_1q_layer.
+ * accumulate = 1 starts from out's current matrices instead of the identity, so
+ * longer lists are fused UA_FUSE_MAX_OPS at a time.  Any of the four dtypes.
+ */
+#define UA_FUSE_MAX_OPS 64
+int ua_fuse_1q_ops(void* out, const void* const* ops, int dtype, int num_ops,
+                   const int* slot, int num_slots, int64_t batch, int64_t op_batch_stride,
+                   int accumulate, ua_stream_t stream);
+
+/*
+ * ua_phase_angle_grad -- the angle gradient of apply_phase for real angles
+ * (torch autograd through exp(-1j * angles) * state, operations.py:15-42):
+ * grad[x] = sum_{r < rows} Im(out[r, x]) Re(g[r, x]) - Re(out[r, x]) Im(g[r, x]),
+ * with out = apply_phase's result and g the incoming gradient, both (rows, cols)
+ * contiguous complex64 / complex128; grad: cols float32 or float64 (grad_dtype).
+ * rows = 1 is the elementwise case; angles broadcast over leading dims sum them.
+ */
+int ua_phase_angle_grad(void* grad, int grad_dtype, const void* g, const void* out, int dtype,
+                        int64_t rows, int64_t cols, ua_stream_t stream);
+
+/*
  * Circuits.  unitair has no circuit object: a circuit is the user's loop of
  * apply_operator / apply_phase calls (unitair README.rst:11-16).  ua_run_circuit
  * takes that loop as data and launches every gate from C onto `stream`.

@@ -255,3 +255,60 @@ extern "C" int ua_apply_phase(void* out, int out_dtype, const void* state, int s
                               rank, shape, state_strides, angle_strides,
                               static_cast<hipStream_t>(stream));
 }
+
+// ---- angle gradient of apply_phase (real angles)
+//
+// torch differentiates exp(-1j*angles) * state with grad_angles =
+// Re(1j * conj(out) * g) summed over the broadcast rows:
+// Im(out) Re(g) - Re(out) Im(g).  One thread per angle column walks the rows
+// (consecutive threads read consecutive amplitudes) and sums in float64.
+namespace ua {
+namespace {
+
+template <typename T, typename G>
+__global__ __launch_bounds__(kBlock) void phase_angle_grad_kernel(G* grad, const T* g,
+                                                                  const T* out, int64_t rows,
+                                                                  int64_t cols) {
+  const int64_t x = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (x >= cols) return;
+  double acc = 0.0;
+  for (int64_t r = 0; r < rows; ++r) {
+    const T o = out[r * cols + x], d = g[r * cols + x];
+    acc += double(o.y) * double(d.x) - double(o.x) * double(d.y);
+  }
+  grad[x] = G(acc);
+}
+
+template <typename T>
+int launch_angle_grad(void* grad, int grad_dtype, const void* g, const void* out, int64_t rows,
+                      int64_t cols, hipStream_t s) {
+  const unsigned grid = grid_for(cols);
+  if (grad_dtype == UA_FLOAT32)
+    phase_angle_grad_kernel<T, float><<<grid, kBlock, 0, s>>>(
+        static_cast<float*>(grad), static_cast<const T*>(g), static_cast<const T*>(out), rows,
+        cols);
+  else
+    phase_angle_grad_kernel<T, double><<<grid, kBlock, 0, s>>>(
+        static_cast<double*>(grad), static_cast<const T*>(g), static_cast<const T*>(out), rows,
+        cols);
+  return check_launch("phase_angle_grad_kernel");
+}
+
+}  // namespace
+}  // namespace ua
+
+extern "C" int ua_phase_angle_grad(void* grad, int grad_dtype, const void* g, const void* out,
+                                   int dtype, int64_t rows, int64_t cols, ua_stream_t stream) {
+  using namespace ua;
+  UA_CHECK_ARG(dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128,
+               "phase angle gradient: complex states only (dtype %d)", dtype);
+  UA_CHECK_ARG(grad_dtype == UA_FLOAT32 || grad_dtype == UA_FLOAT64,
+               "phase angle gradient: real angles only (dtype %d)", grad_dtype);
+  UA_CHECK_ARG(rows >= 0 && cols >= 0, "negative size");
+  if (cols == 0) return UA_OK;
+  UA_CHECK_ARG(grad && (rows == 0 || (g && out)), "NULL pointer");
+  UA_CHECK_ARG(cols / kBlock < 0x7fffffffLL, "too many angles");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == UA_COMPLEX64) return launch_angle_grad<float2>(grad, grad_dtype, g, out, rows, cols, s);
+  return launch_angle_grad<double2>(grad, grad_dtype, g, out, rows, cols, s);
+}

@@ -124,6 +124,10 @@ SIGNATURES["ua_run_fused_pass"] = (_i32, [_p, _i32, _i32, _i64, ctypes.POINTER(F
 SIGNATURES["ua_run_circuit"] = (_i32, [_p, _p, _i32, _i32, _i64, ctypes.POINTER(Gate), _i32,
                                        ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.POINTER(ctypes.c_int), _p])
+UA_FUSE_MAX_OPS = 64
+SIGNATURES["ua_fuse_1q_ops"] = (_i32, [_p, ctypes.POINTER(ctypes.c_void_p), _i32, _i32,
+                                       ctypes.POINTER(ctypes.c_int), _i32, _i64, _i64, _i32, _p])
+SIGNATURES["ua_phase_angle_grad"] = (_i32, [_p, _i32, _p, _p, _i32, _i64, _i64, _p])
 SIGNATURES["ua_plan_dense_blocks"] = (_i32, [_i32, _i32, ctypes.POINTER(Gate), _i32,
                                              ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(ctypes.c_int)])

@@ -339,8 +339,16 @@ def apply_to_qubits(operators: Iterable[torch.Tensor], qubits: Iterable[int],
     entry by entry; other broadcasts apply the fused gates one by one with the
     reference's apply_operator rules.
     """
-    from ..gates.matrix_algebra import fuse_single_qubit_operators
+    from ..gates.matrix_algebra import _native_ok, fuse_1q_native, fuse_single_qubit_operators
     n = count_qubits(state)
+    qubits, operators = list(qubits), list(operators)
+    raw = operators[:len(qubits)]
+    if (raw and not _needs_grad(state) and _native_ok(raw) and
+            raw[0].dtype == state.dtype and raw[0].device == state.device and
+            all(0 <= int(q) < n for q in qubits[:len(raw)])):
+        # no autograd: the gates on each qubit are multiplied by one native
+        # launch straight into the 1-qubit layer's operand layout
+        return _apply_1q_layer_fused(raw, [int(q) for q in qubits[:len(raw)]], state, n)
     fused = fuse_single_qubit_operators(qubits, operators)
     for q in fused:
         if not (0 <= int(q) < n):
@@ -386,6 +394,55 @@ def apply_to_qubits(operators: Iterable[torch.Tensor], qubits: Iterable[int],
         rc = lib.ua_apply_1q_layer(
             _lib.ptr(out), _lib.ptr(s_flat), _lib.ptr(mats), _lib.DTYPE_CODES[state.dtype], n,
             k, _lib.int_array(qs), batch, s_stride, 4 * k if obatch else 0,
+            _lib.stream_handle(state.device))
+    _lib.check(rc, "apply_to_qubits")
+    return out
+
+
+def _apply_1q_layer_fused(ops: List[torch.Tensor], qubits: List[int], state: torch.Tensor,
+                          n: int) -> torch.Tensor:
+    """apply_to_qubits without autograd: ua_fuse_1q_ops multiplies the gates of
+    each qubit into a (*batch, k, 2, 2) operand, then ua_apply_1q_layer runs the
+    layer.  ops: equal-shape (*obatch, 2, 2) device tensors of the state's dtype."""
+    from ..gates.matrix_algebra import fuse_1q_native
+    for op in ops:
+        _check_operator(op, state, "apply_to_qubits")
+    slot_of = {}
+    for q in qubits:
+        slot_of.setdefault(q, len(slot_of))
+    qs = list(slot_of)
+    obatch = tuple(ops[0].shape[:-2])
+    sbatch = tuple(state.shape[:-1])
+    if not (obatch == () or obatch == sbatch or sbatch == ()):
+        # other broadcasts: the reference's per-gate rules on the fused gates
+        fused = torch.empty(obatch + (len(qs), 2, 2), dtype=state.dtype, device=state.device)
+        fuse_1q_native([prepare(o).contiguous() for o in ops], [slot_of[q] for q in qubits],
+                       len(qs), fused)
+        x = state
+        for s, q in enumerate(qs):
+            x = apply_operator(fused[..., s, :, :], [q], x)
+        return x
+    if state.dtype not in _SUPPORTED:
+        raise TypeError(f"apply_to_qubits: dtype {state.dtype} is not supported")
+    state = prepare(state)
+    N = 2 ** n
+    if obatch and not sbatch:
+        out_batch, s_flat, s_stride = obatch, state.reshape(N).contiguous(), 0
+    else:
+        out_batch = sbatch
+        s_flat, s_stride = _flat_batch(state, N)
+    batch = math.prod(out_batch)
+    out = torch.empty(tuple(out_batch) + (N,), dtype=state.dtype, device=state.device)
+    if batch == 0:
+        return out
+    mats = torch.empty(obatch + (len(qs), 2, 2), dtype=state.dtype, device=state.device)
+    fuse_1q_native([prepare(o).contiguous() for o in ops], [slot_of[q] for q in qubits],
+                   len(qs), mats)
+    lib = _lib.load()
+    with on_device(state.device):
+        rc = lib.ua_apply_1q_layer(
+            _lib.ptr(out), _lib.ptr(s_flat), _lib.ptr(mats), _lib.DTYPE_CODES[state.dtype], n,
+            len(qs), _lib.int_array(qs), batch, s_stride, 4 * len(qs) if obatch else 0,
             _lib.stream_handle(state.device))
     _lib.check(rc, "apply_to_qubits")
     return out
@@ -622,10 +679,41 @@ class _ApplyPhase(torch.autograd.Function):
             gs = gs if state.is_complex() else gs.real
             gs = gs.to(state.dtype)
         if ctx.needs_input_grad[0]:
-            full = 1j * out.conj() * g
-            ga = full.sum_to_size(angles.shape)
-            ga = (ga if angles.is_complex() else ga.real).to(angles.dtype)
+            ga = _phase_angle_grad(angles, out, g)
+            if ga is None:   # complex angles or an unusual broadcast: torch's rule as is
+                full = 1j * out.conj() * g
+                ga = full.sum_to_size(angles.shape)
+                ga = (ga if angles.is_complex() else ga.real).to(angles.dtype)
         return ga, gs
+
+
+def _phase_angle_grad(angles: torch.Tensor, out: torch.Tensor, g: torch.Tensor):
+    """grad of real angles through exp(-1j*angles)*state by ua_phase_angle_grad:
+    Im(out) Re(g) - Re(out) Im(g), summed over the leading dims that the angles
+    broadcast over.  None for the cases it does not cover."""
+    if angles.is_complex() or angles.dtype not in (torch.float32, torch.float64) or \
+            out.dtype not in (torch.complex64, torch.complex128) or g.dtype != out.dtype or \
+            g.shape != out.shape or angles.numel() == 0:
+        return None
+    # the angles' shape, leading 1s aside, must be a trailing block of the output's
+    a_shape = tuple(angles.shape)
+    while a_shape and a_shape[0] == 1:
+        a_shape = a_shape[1:]
+    o_shape = tuple(out.shape)
+    if len(a_shape) > len(o_shape) or o_shape[len(o_shape) - len(a_shape):] != a_shape:
+        return None
+    cols = math.prod(a_shape)
+    rows = out.numel() // cols if cols else 0
+    grad = torch.empty(angles.shape, dtype=angles.dtype, device=angles.device)
+    lib = _lib.load()
+    with on_device(out.device):
+        rc = lib.ua_phase_angle_grad(_lib.ptr(grad), _lib.DTYPE_CODES[angles.dtype],
+                                     _lib.ptr(prepare(g).contiguous()),
+                                     _lib.ptr(prepare(out).contiguous()),
+                                     _lib.DTYPE_CODES[out.dtype], rows, cols,
+                                     _lib.stream_handle(out.device))
+    _lib.check(rc, "phase_angle_grad")
+    return grad
 
 
 def apply_phase(angles: torch.Tensor, state: torch.Tensor) -> torch.Tensor:

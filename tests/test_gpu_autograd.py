@@ -80,7 +80,8 @@ def test_operator_gradient_complex64_large(device):
         assert float((r - g.to(torch.complex128)).abs().max()) <= 1e-5 * scale
 
 
-@pytest.mark.parametrize("ashape,sshape", [((16,), (3, 16)), ((3, 16), (3, 16)), ((3, 1), (3, 16))])
+@pytest.mark.parametrize("ashape,sshape", [((16,), (3, 16)), ((3, 16), (3, 16)), ((3, 1), (3, 16)),
+                                           ((), (3, 16)), ((1, 16), (2, 3, 16))])
 def test_apply_phase_autograd(device, ashape, sshape):
     torch.manual_seed(2)
     ang = torch.rand(ashape, dtype=torch.float64) * 6
@@ -89,6 +90,18 @@ def test_apply_phase_autograd(device, ashape, sshape):
     got = _grads_gpu(lambda a, s: sim.apply_phase(a, s), device, ang, st)
     for r, g in zip(ref, got):
         assert torch.allclose(r, g, atol=1e-10)
+
+
+def test_apply_phase_angle_gradient_complex64(device):
+    """The native angle gradient (float32 angles, complex64 state) against the
+    float64 autograd of the restatement."""
+    torch.manual_seed(4)
+    ang = torch.rand(2 ** 12, dtype=torch.float32) * 6
+    st = torch.randn(5, 2 ** 12, dtype=torch.complex64)
+    ref = _grads_cpu(lambda a, s: T.apply_phase(a, s), ang.double(), st.to(torch.complex128))
+    got = _grads_gpu(lambda a, s: sim.apply_phase(a, s), device, ang, st)
+    assert got[0].dtype == torch.float32
+    assert float((ref[0] - got[0].double()).abs().max()) < 1e-4 * float(ref[0].abs().max())
 
 
 def test_apply_all_qubits_autograd(device):

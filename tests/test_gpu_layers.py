@@ -166,6 +166,43 @@ def test_fuse_single_qubit_operators_on_device(device):
     assert torch.allclose(f[1], C @ A) and f[0] is B
 
 
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128, torch.float32, torch.float64])
+@pytest.mark.parametrize("batch,count", [((), 7), ((3,), 5), ((2, 2), 150)])
+def test_fuse_single_qubit_operators_native_products(device, dt, batch, count):
+    """ua_fuse_1q_ops (chained past 64 operators) against products on the host."""
+    rng = np.random.default_rng(count + len(batch))
+    qubits = [int(q) for q in rng.integers(0, 4, count)]
+    npdt = {torch.complex64: np.complex64, torch.complex128: np.complex128,
+            torch.float32: np.float32, torch.float64: np.float64}[dt]
+    ops = [rand(batch + (2, 2), dt, rng).astype(npdt) for _ in range(count)]
+    f = fuse_single_qubit_operators(qubits, [torch.from_numpy(o).to(device) for o in ops])
+    ref = {}
+    for q, o in zip(qubits, ops):
+        o = o.astype(np.complex128)
+        ref[q] = o @ ref[q] if q in ref else o
+    assert list(f) == list(ref)
+    tol = 1e-4 if dt in (torch.complex64, torch.float32) else 1e-11
+    for q in ref:
+        got = f[q].cpu().numpy().astype(np.complex128)
+        assert got.shape == ref[q].shape
+        assert np.max(np.abs(got - ref[q])) <= tol * max(1.0, np.max(np.abs(ref[q])))
+
+
+def test_apply_to_qubits_native_fusion_equals_gate_by_gate(device):
+    """The fused launch (ua_fuse_1q_ops + ua_apply_1q_layer) against the
+    reference's gate-by-gate rule, batched operators, many repeats."""
+    rng = np.random.default_rng(31)
+    n = 14
+    qubits = [int(q) for q in rng.integers(0, n, 90)]
+    ops = [cast(rand((2, 2, 2), torch.complex128, rng), torch.complex128) for _ in qubits]
+    st = cast(rand((2, 2 ** n), torch.complex128, rng), torch.complex128)
+    out = sim.apply_to_qubits([torch.from_numpy(o).to(device) for o in ops], qubits,
+                              torch.from_numpy(st).to(device)).cpu().numpy()
+    for r in range(2):
+        ref = U.apply_to_qubits([o[r] for o in ops], qubits, st[r])
+        assert rel(out[r], ref) < 1e-11
+
+
 def test_act_last_qubit(device):
     # reference tests/test_unitary.py:54-92
     rng = np.random.default_rng(8)
