@@ -181,17 +181,75 @@ __device__ __forceinline__ void dispatch_2q(T (&c)[kCube], int b1, int b0, P u) 
   }
 }
 
-// swap cube entries that differ in cube bit B where `ctrl` has bit j set
+// Component-wise s ? a : b (a select of two vector structs is lowered to a
+// select of their addresses, which keeps the cube array out of registers).
+template <typename T>
+__device__ __forceinline__ T csel(bool s, T a, T b) {
+  T r;
+  r.x = s ? a.x : b.x;
+  r.y = s ? a.y : b.y;
+  return r;
+}
+
+// ---- sign, swap and diagonal steps specialised by where their qubits sit.
+// Both qubits in the cube give a fixed pattern of entries (no per-entry test);
+// a qubit outside the cube is one bit per thread, so the step is one factor
+// or one select for the whole cube.
+
+// negate the entries whose cube bits A and B are both set
+template <int A, int B, typename T>
+__device__ __forceinline__ void negate_cc(T (&c)[kCube]) {
+#pragma unroll
+  for (int j = 0; j < kCube; ++j)
+    if (((j >> A) & 1) && ((j >> B) & 1)) c[j] = cneg(c[j]);
+}
+// multiply the entries whose cube bit A is set by s (+1 or -1)
+template <int A, typename T, typename R>
+__device__ __forceinline__ void scale_c(T (&c)[kCube], R s) {
+#pragma unroll
+  for (int j = 0; j < kCube; ++j)
+    if ((j >> A) & 1) {
+      c[j].x *= s;
+      c[j].y *= s;
+    }
+}
+template <typename T, typename R>
+__device__ __forceinline__ void scale_all(T (&c)[kCube], R s) {
+#pragma unroll
+  for (int j = 0; j < kCube; ++j) {
+    c[j].x *= s;
+    c[j].y *= s;
+  }
+}
+// CNOT with the control on cube bit C and the target on cube bit B: swap the
+// pairs whose control bit is set
+template <int C, int B, typename T>
+__device__ __forceinline__ void cnot_cc(T (&c)[kCube]) {
+#pragma unroll
+  for (int j = 0; j < kCube; ++j)
+    if (((j >> C) & 1) && !((j >> B) & 1)) {
+      const T x = c[j];
+      c[j] = c[j | (1 << B)];
+      c[j | (1 << B)] = x;
+    }
+}
+// CNOT with the control outside the cube (this thread's bit s): swap every
+// pair along cube bit B, or none
 template <int B, typename T>
-__device__ __forceinline__ void cube_cnot(T (&c)[kCube], uint32_t ctrl) {
+__device__ __forceinline__ void cnot_lane(T (&c)[kCube], bool s) {
 #pragma unroll
   for (int j = 0; j < kCube; ++j) {
     if (j & (1 << B)) continue;
-    const bool s = (ctrl >> j) & 1;
     const T x = c[j], y = c[j | (1 << B)];
-    c[j] = s ? y : x;
-    c[j | (1 << B)] = s ? x : y;
+    c[j] = csel(s, y, x);
+    c[j | (1 << B)] = csel(s, x, y);
   }
+}
+// diag(d0, d1) on cube bit A
+template <int A, typename T>
+__device__ __forceinline__ void diag_c(T (&c)[kCube], T d0, T d1) {
+#pragma unroll
+  for (int j = 0; j < kCube; ++j) c[j] = cmul(((j >> A) & 1) ? d1 : d0, c[j]);
 }
 
 template <typename T>
@@ -482,14 +540,12 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd<T>()))) void fused_pass_kernel
       for (int i = 0; i < m; ++i) gbase |= uint64_t((e0 >> i) & 1) << a.local[i];
     }
     const int cb0 = a.local[s0], cb1 = a.local[s1], cb2 = a.local[s2], cb3 = a.local[s3];
-    // 16-bit mask of the cube entries whose state index has bit `bit` set
-    auto sel = [&](int bit) -> uint32_t {
-      if (bit == cb0) return 0xAAAAu;
-      if (bit == cb1) return 0xCCCCu;
-      if (bit == cb2) return 0xF0F0u;
-      if (bit == cb3) return 0xFF00u;
-      return ((gbase >> bit) & 1) ? 0xFFFFu : 0u;
+    // cube position (0..3) of state bit `bit`, or -1 (wave-uniform)
+    auto slot_of = [&](int bit) -> int {
+      return bit == cb0 ? 0 : bit == cb1 ? 1 : bit == cb2 ? 2 : bit == cb3 ? 3 : -1;
     };
+    // this thread's value of a state bit outside the cube
+    auto bit_of = [&](int bit) -> bool { return (gbase >> bit) & 1; };
     if (active) {
       for (int o = a.stage_ops[st]; o < a.stage_ops[st + 1]; ++o) {
         const ua_fused_op op = load_op(ops + o);
@@ -504,27 +560,77 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd<T>()))) void fused_pass_kernel
           }
           case UA_FOP_CNOT: {  // a: control state bit, b: target cube bit
             if constexpr (VARIANT != kLayers) {
-              const uint32_t ctrl = sel(op.a);
-              switch (op.b) {
-                case 0: cube_cnot<0>(q, ctrl); break;
-                case 1: cube_cnot<1>(q, ctrl); break;
-                case 2: cube_cnot<2>(q, ctrl); break;
-                default: cube_cnot<3>(q, ctrl); break;
+              const int kc = slot_of(op.a);
+              if (kc >= 0) {
+                switch (kc * 4 + op.b) {
+                  case 1: cnot_cc<0, 1>(q); break;
+                  case 2: cnot_cc<0, 2>(q); break;
+                  case 3: cnot_cc<0, 3>(q); break;
+                  case 4: cnot_cc<1, 0>(q); break;
+                  case 6: cnot_cc<1, 2>(q); break;
+                  case 7: cnot_cc<1, 3>(q); break;
+                  case 8: cnot_cc<2, 0>(q); break;
+                  case 9: cnot_cc<2, 1>(q); break;
+                  case 11: cnot_cc<2, 3>(q); break;
+                  case 12: cnot_cc<3, 0>(q); break;
+                  case 13: cnot_cc<3, 1>(q); break;
+                  default: cnot_cc<3, 2>(q); break;
+                }
+              } else {
+                const bool s = bit_of(op.a);
+                switch (op.b) {
+                  case 0: cnot_lane<0>(q, s); break;
+                  case 1: cnot_lane<1>(q, s); break;
+                  case 2: cnot_lane<2>(q, s); break;
+                  default: cnot_lane<3>(q, s); break;
+                }
               }
             }
             break;
           }
           case UA_FOP_CZ: {  // a, b: state bits
-            if constexpr (VARIANT != kLayers) negate_where(q, sel(op.a) & sel(op.b));
+            if constexpr (VARIANT != kLayers) {
+              const int ka = slot_of(op.a), kb = slot_of(op.b);
+              if (ka >= 0 && kb >= 0) {
+                switch ((1 << ka) | (1 << kb)) {
+                  case 3: negate_cc<0, 1>(q); break;
+                  case 5: negate_cc<0, 2>(q); break;
+                  case 9: negate_cc<0, 3>(q); break;
+                  case 6: negate_cc<1, 2>(q); break;
+                  case 10: negate_cc<1, 3>(q); break;
+                  default: negate_cc<2, 3>(q); break;
+                }
+              } else if (ka >= 0 || kb >= 0) {
+                const int k = ka >= 0 ? ka : kb;
+                const R sg = bit_of(ka >= 0 ? op.b : op.a) ? R(-1) : R(1);
+                switch (k) {
+                  case 0: scale_c<0>(q, sg); break;
+                  case 1: scale_c<1>(q, sg); break;
+                  case 2: scale_c<2>(q, sg); break;
+                  default: scale_c<3>(q, sg); break;
+                }
+              } else {
+                scale_all(q, (bit_of(op.a) && bit_of(op.b)) ? R(-1) : R(1));
+              }
+            }
             break;
           }
           case UA_FOP_DIAG1: {  // a: state bit; data: (d0, d1)
             if constexpr (VARIANT != kLayers) {
               const auto d = rows(op, b_c);
               const T d0 = ldc<T>(d, 0), d1 = ldc<T>(d, 1);
-              const uint32_t one = sel(op.a);
+              const int k = slot_of(op.a);
+              switch (k) {
+                case 0: diag_c<0>(q, d0, d1); break;
+                case 1: diag_c<1>(q, d0, d1); break;
+                case 2: diag_c<2>(q, d0, d1); break;
+                case 3: diag_c<3>(q, d0, d1); break;
+                default: {
+                  const T dd = csel(bit_of(op.a), d1, d0);
 #pragma unroll
-              for (int j = 0; j < kCube; ++j) q[j] = cmul(((one >> j) & 1) ? d1 : d0, q[j]);
+                  for (int j = 0; j < kCube; ++j) q[j] = cmul(dd, q[j]);
+                }
+              }
             }
             break;
           }
