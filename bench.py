@@ -496,7 +496,7 @@ def _op_table(n, sim, states, psi, other, u, ang, diag, sample, rand_state):
          lambda: sim.apply_operator(u[2], [5, 17], psi), 2 * A * s),
         ("apply_operator 5q Haar, qubits (0..4)",
          lambda: sim.apply_operator(u[5], [0, 1, 2, 3, 4], psi), 2 * A * s),
-        ("apply_operator 6q Haar, qubits (0, 7, 12, 3, 23, 18) (tiled wide-gate kernel)",
+        ("apply_operator 6q Haar, qubits (0, 7, 12, 3, 23, 18) (matrix cores, one-gate pass)",
          lambda: sim.apply_operator(u[6], [0, 7, 12, 3, 23, 18], psi), 2 * A * s),
         ("apply_operator 8q Haar, qubits (2, 9, 14, 0, 21, 5, 17, 11) (tiled wide-gate kernel)",
          lambda: sim.apply_operator(u[8], [2, 9, 14, 0, 21, 5, 17, 11], psi), 2 * A * s),

@@ -22,7 +22,9 @@
 // A pass holds up to 3 gates in complex64 (80 KB of LDS, two workgroups per
 // CU) and 2 in complex128 (128 KB); the union of two 5-qubit gates always
 // fits (10 bits + 2).  Gates of other kinds or sizes, and states below 14
-// qubits, run one by one as ua_run_circuit runs them.
+// qubits, run one by one as ua_run_circuit runs them.  The same kernel, with
+// a 64 KB A-operand block, takes single 6-qubit complex64 gates of
+// apply_operator (in place or not).
 #include "mfma_common.h"
 
 namespace ua {
@@ -37,18 +39,20 @@ constexpr int kTile = 1 << kTB;
 constexpr int kMaxPassGates = 3;
 constexpr int kMinBlockQubits = 14;   // whole tiles and a few of them per state
 constexpr int kFragWords = 4096;      // A-operand reals of a 5-qubit gate (64 x 64)
+constexpr int kFragWords6 = 16384;    // of a 6-qubit gate (128 x 128)
 
 struct BlockGate {
   const void* op;   // D x D matrix, row-major, in the caller's qubit order
-  int k;            // 4 or 5
-  int tpos[5];      // tile positions of the targets, ascending (canonical order)
-  int gpos[5];      // gate-index bit of canonical target m
+  int k;            // 4, 5 (several per pass) or 6 (alone, complex64)
+  int tpos[6];      // tile positions of the targets, ascending (canonical order)
+  int gpos[6];      // gate-index bit of canonical target m
   int qpos[8];      // tile positions of the group bits, ascending (12 - k of them)
 };
 
 struct BlockPass {
   int64_t tiles;        // batch * 2^(n - 12)
-  int64_t bstride;      // 2^n
+  int64_t in_bstride;   // batch stride of the input (0: one state for every entry)
+  int64_t bstride;      // batch stride of the output: 2^n
   int log2_tiles;       // n - 12
   int xcd;              // visit tiles in XCD-contiguous ranges
   int local[kTB];       // ascending state bits of the tile
@@ -96,24 +100,32 @@ __device__ __forceinline__ void apply_gate(typename RealOf<T>::type* xr,
   constexpr int KS = 2 * D / 4;            // k-steps
   constexpr int AV = 16 / sizeof(R);       // A values per 16-byte read
   constexpr int CT = (kTile >> K) / 16;    // column tiles (16 groups each)
-  constexpr int CPW = CT / W;              // column tiles per wavefront
-  static_assert(CPW >= 1 && CT % W == 0, "column tiles per wavefront");
+  // With fewer column tiles than wavefronts, WPC wavefronts share a column
+  // tile, each taking RTW of its row tiles (then the results are written
+  // only after every wavefront has read its B-operands).
+  constexpr int WPC = W > CT ? W / CT : 1;
+  constexpr int CPW = W > CT ? 1 : CT / W;  // column tiles per wavefront
+  constexpr int RTW = RT / WPC;             // row tiles per wavefront
+  static_assert(CT % (W / WPC) == 0 && RT % WPC == 0, "wavefront split");
+  static_assert(WPC == 1 || RTW <= RT / 2, "a wavefront's rows stay in one half");
   const int nq = kTB - K;
+  const int cw = wave % (W / WPC);          // column-tile lane of this wavefront
+  const int t0 = (wave / (W / WPC)) * RTW;  // first row tile
   bl = opaque(bl);
   int cb[CPW];  // swizzled offsets of this wavefront's column tiles
 #pragma unroll
-  for (int c = 0; c < CPW; ++c) cb[c] = swz(deposit(16 * (wave + W * c), g.qpos, nq));
-  typename M::acc_t acc[CPW][RT];
+  for (int c = 0; c < CPW; ++c) cb[c] = swz(deposit(16 * (cw + (W / WPC) * c), g.qpos, nq));
+  typename M::acc_t acc[CPW][RTW];
 #pragma unroll
   for (int c = 0; c < CPW; ++c)
 #pragma unroll
-    for (int ti = 0; ti < RT; ++ti) acc[c][ti] = typename M::acc_t{0, 0, 0, 0};
+    for (int ti = 0; ti < RTW; ++ti) acc[c][ti] = typename M::acc_t{0, 0, 0, 0};
 #pragma unroll
   for (int kk = 0; kk < KS / AV; ++kk) {
-    V a[RT];
+    V a[RTW];
 #pragma unroll
-    for (int ti = 0; ti < RT; ++ti)
-      a[ti] = *reinterpret_cast<const V*>(af + ((ti * (KS / AV) + kk) * 64 + lane) * AV);
+    for (int ti = 0; ti < RTW; ++ti)
+      a[ti] = *reinterpret_cast<const V*>(af + (((t0 + ti) * (KS / AV) + kk) * 64 + lane) * AV);
 #pragma unroll
     for (int q = 0; q < AV; ++q) {
       const int ks = kk * AV + q;
@@ -123,32 +135,38 @@ __device__ __forceinline__ void apply_gate(typename RealOf<T>::type* xr,
 #pragma unroll
       for (int c = 0; c < CPW; ++c) b[c] = src[bl ^ sj ^ cb[c]];
 #pragma unroll
-      for (int ti = 0; ti < RT; ++ti)
+      for (int ti = 0; ti < RTW; ++ti)
 #pragma unroll
         for (int c = 0; c < CPW; ++c) acc[c][ti] = M::mma(a[ti][q], b[c], acc[c][ti]);
     }
   }
+  if constexpr (WPC > 1) __syncthreads();  // every B-operand of the shared columns is read
   int w[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) w[r] = opaque(wl[r]);
 #pragma unroll
-  for (int ti = 0; ti < RT; ++ti) {
+  for (int tl = 0; tl < RTW; ++tl) {
+    const int ti = t0 + tl;
     R* dst = (ti >= RT / 2) ? xi : xr;
     const int sj = swz(deposit(16 * (ti % (RT / 2)), g.tpos, K));
 #pragma unroll
     for (int c = 0; c < CPW; ++c)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) dst[w[r] ^ sj ^ cb[c]] = acc[c][ti][r];
+      for (int r = 0; r < 4; ++r) dst[w[r] ^ sj ^ cb[c]] = acc[c][tl][r];
   }
 }
 
-template <typename T, int THREADS, int MAXG>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) void dense_blocks_kernel(T* state, BlockPass p) {
+// FRAG: A-operand reals per gate (kFragWords for 4/5-qubit gates, kFragWords6
+// for one 6-qubit gate).  `in` may equal `out` (in place): every tile is read
+// before it is written, and tiles are disjoint.
+template <typename T, int THREADS, int MAXG, int FRAG>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FRAG == kFragWords6 ? 1 : 2))) void dense_blocks_kernel(
+    T* out, const T* in, BlockPass p) {
   using R = typename RealOf<T>::type;
   constexpr int UPT = kTile / 2 / THREADS;  // element pairs per thread
   constexpr int AV = 16 / sizeof(R);
   __shared__ R xr[kTile], xi[kTile];
-  __shared__ __attribute__((aligned(16))) R af[MAXG][kFragWords];
+  __shared__ __attribute__((aligned(16))) R af[MAXG][FRAG];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   // ---- A-operands of every gate, layout [ti][ks / AV][lane][ks % AV]
@@ -203,18 +221,18 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
     const int64_t t = blockIdx.x + it * int64_t(gridDim.x);
     return t < p.tiles ? t : -1;
   };
-  auto origin_of = [&](int64_t t) -> int64_t {
+  auto origin_of = [&](int64_t t, int64_t bstride) -> int64_t {
     const int64_t b = t >> p.log2_tiles;
     const uint64_t tin = uint64_t(t) & ((uint64_t(1) << p.log2_tiles) - 1);
-    return b * p.bstride + int64_t(insert_zero_bits(tin, p.local, kTB));
+    return b * bstride + int64_t(insert_zero_bits(tin, p.local, kTB));
   };
 
   Pair<T> v[UPT];
   int64_t tile = tile_at(0);
   if (tile >= 0) {
-    const int64_t o = origin_of(tile) + dep_t;
+    const int64_t o = origin_of(tile, p.in_bstride) + dep_t;
 #pragma unroll
-    for (int u = 0; u < UPT; ++u) v[u] = *reinterpret_cast<const Pair<T>*>(state + o + dep_u(u));
+    for (int u = 0; u < UPT; ++u) v[u] = *reinterpret_cast<const Pair<T>*>(in + o + dep_u(u));
   }
   for (int64_t it = 0; tile >= 0; ++it) {
     const int st = opaque(swz_t);
@@ -229,22 +247,24 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
     __syncthreads();
     const int64_t next = tile_at(it + 1);
     if (next >= 0) {
-      const int64_t o = origin_of(next) + dep_t;
+      const int64_t o = origin_of(next, p.in_bstride) + dep_t;
 #pragma unroll
       for (int u = 0; u < UPT; ++u)
-        v[u] = *reinterpret_cast<const Pair<T>*>(state + o + dep_u(u));
+        v[u] = *reinterpret_cast<const Pair<T>*>(in + o + dep_u(u));
     }
 #pragma unroll
     for (int gi = 0; gi < MAXG; ++gi) {
       if (gi < p.num_gates) {
-        if (p.g[gi].k == 5)
+        if constexpr (FRAG == kFragWords6)
+          apply_gate<T, THREADS, 6>(xr, xi, af[gi], p.g[gi], lane, wave, bl[gi], wl[gi]);
+        else if (p.g[gi].k == 5)
           apply_gate<T, THREADS, 5>(xr, xi, af[gi], p.g[gi], lane, wave, bl[gi], wl[gi]);
         else
           apply_gate<T, THREADS, 4>(xr, xi, af[gi], p.g[gi], lane, wave, bl[gi], wl[gi]);
         __syncthreads();
       }
     }
-    const int64_t o = origin_of(tile) + dep_t;
+    const int64_t o = origin_of(tile, p.bstride) + dep_t;
     const int st2 = opaque(swz_t);
 #pragma unroll
     for (int u = 0; u < UPT; ++u) {
@@ -254,7 +274,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
       w.v[0].y = xi[e];
       w.v[1].x = xr[e ^ 1];
       w.v[1].y = xi[e ^ 1];
-      *reinterpret_cast<Pair<T>*>(state + o + dep_u(u)) = w;
+      *reinterpret_cast<Pair<T>*>(out + o + dep_u(u)) = w;
     }
     __syncthreads();
     tile = next;
@@ -264,6 +284,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(2))) vo
 // Launch geometry: complex64 256 threads and 3 gates (80 KB LDS, two
 // workgroups per CU); complex128 512 threads and 2 gates (128 KB, one).
 constexpr int kThreads64 = 256, kThreads128 = 512;
+constexpr int kThreads6 = 512;  // 6-qubit gates: 4 column tiles x 2 row halves
 constexpr int kMaxGates64 = 3, kMaxGates128 = 2;
 constexpr int kBlocksPerCu64 = 2, kBlocksPerCu128 = 1;
 constexpr int kCus = 256;
@@ -347,26 +368,33 @@ int check_gates(int n, const ua_gate* gates, int num_gates) {
   return UA_OK;
 }
 
-int launch_pass(void* state, int dtype, int n, int64_t batch, const ua_gate* gates,
-                const Unit& u, hipStream_t s) {
+// One pass of gates[0..count) (4/5-qubit gates, or a single 6-qubit complex64
+// gate) from `in` (batch stride in_bstride) to `out` (stride 2^n; may be `in`).
+int launch_pass(void* out, const void* in, int64_t in_bstride, int dtype, int n, int64_t batch,
+                const ua_gate* gates, int count, hipStream_t s) {
   BlockPass p{};
   p.log2_tiles = n - kTB;
   p.tiles = batch << p.log2_tiles;
+  p.in_bstride = in_bstride;
   p.bstride = int64_t(1) << n;
   uint64_t uni = 3;
-  for (int i = 0; i < u.count; ++i) uni |= target_mask(gates[u.first + i], n);
+  for (int i = 0; i < count; ++i) uni |= target_mask(gates[i], n);
+  UA_CHECK_ARG(__builtin_popcountll(uni) <= kTB, "internal: %d tile bits",
+               __builtin_popcountll(uni));
   for (int bit = 0; __builtin_popcountll(uni) < kTB; ++bit) uni |= uint64_t(1) << bit;
   for (int i = 0; i < kTB; ++i) {
     p.local[i] = __builtin_ctzll(uni);
     uni &= uni - 1;
   }
-  p.num_gates = u.count;
-  for (int gi = 0; gi < u.count; ++gi) {
-    const ua_gate& g = gates[u.first + gi];
+  p.num_gates = count;
+  bool six = false;
+  for (int gi = 0; gi < count; ++gi) {
+    const ua_gate& g = gates[gi];
     BlockGate& b = p.g[gi];
     b.op = g.data;
     b.k = g.num_targets;
-    int tb[5], gp[5];
+    six |= b.k == 6;
+    int tb[6], gp[6];
     for (int m = 0; m < b.k; ++m) {
       tb[m] = n - 1 - g.qubits[m];
       gp[m] = b.k - 1 - m;
@@ -391,21 +419,48 @@ int launch_pass(void* state, int dtype, int n, int64_t batch, const ua_gate* gat
       if (!((tmask >> i) & 1)) b.qpos[nq++] = i;
   }
   const bool c64 = dtype == UA_COMPLEX64;
-  const int64_t want = int64_t(kCus) * (c64 ? kBlocksPerCu64 : kBlocksPerCu128);
+  UA_CHECK_ARG(!six || (c64 && count == 1), "internal: 6-qubit passes hold one complex64 gate");
+  // 6-qubit gates: 96 KB of LDS (tile + the 128 x 128 block), one workgroup per CU
+  const int per_cu = six ? 1 : (c64 ? kBlocksPerCu64 : kBlocksPerCu128);
+  const int64_t want = int64_t(kCus) * per_cu;
   p.xcd = p.tiles % 8 == 0 && p.tiles >= want;
   int64_t grid = p.tiles < want ? p.tiles : want;
   if (p.xcd) grid &= ~int64_t(7);
   UA_CHECK_ARG(grid >= 1 && grid <= 0x7fffffffLL, "internal: grid %lld", (long long)grid);
-  if (c64)
-    dense_blocks_kernel<float2, kThreads64, kMaxGates64>
-        <<<unsigned(grid), kThreads64, 0, s>>>(static_cast<float2*>(state), p);
+  if (six)
+    dense_blocks_kernel<float2, kThreads6, 1, kFragWords6><<<unsigned(grid), kThreads6, 0, s>>>(
+        static_cast<float2*>(out), static_cast<const float2*>(in), p);
+  else if (c64)
+    dense_blocks_kernel<float2, kThreads64, kMaxGates64, kFragWords>
+        <<<unsigned(grid), kThreads64, 0, s>>>(static_cast<float2*>(out),
+                                               static_cast<const float2*>(in), p);
   else
-    dense_blocks_kernel<double2, kThreads128, kMaxGates128>
-        <<<unsigned(grid), kThreads128, 0, s>>>(static_cast<double2*>(state), p);
+    dense_blocks_kernel<double2, kThreads128, kMaxGates128, kFragWords>
+        <<<unsigned(grid), kThreads128, 0, s>>>(static_cast<double2*>(out),
+                                                static_cast<const double2*>(in), p);
   return check_launch("dense_blocks_kernel");
 }
 
 }  // namespace
+
+// A single 6-qubit complex64 gate of apply_operator on the matrix cores (one
+// operator for every batch entry).  qubits: the caller's order (first = gate
+// MSB).  True when it applies: complex64, at least kMinBlockQubits qubits,
+// pair-aligned buffers and strides.
+bool six_applies(int dtype, int n, int k, const void* out, const void* in, int64_t in_bstride) {
+  return k == 6 && dtype == UA_COMPLEX64 && n >= kMinBlockQubits && pair_aligned(out, dtype) &&
+         pair_aligned(in, dtype) && in_bstride % 2 == 0;
+}
+int apply_dense_six(void* out, const void* in, const void* op, int n, const int* qubits,
+                    int64_t batch, int64_t in_bstride, hipStream_t s) {
+  ua_gate g{};
+  g.kind = UA_GATE_DENSE;
+  g.num_targets = 6;
+  for (int m = 0; m < 6; ++m) g.qubits[m] = qubits[m];
+  g.data = op;
+  return launch_pass(out, in, in_bstride, UA_COMPLEX64, n, batch, &g, 1, s);
+}
+
 }  // namespace ua
 
 extern "C" int ua_plan_dense_blocks(int dtype, int num_qubits, const ua_gate* gates,
@@ -448,7 +503,8 @@ extern "C" int ua_run_dense_blocks(void* state, int dtype, int num_qubits, int64
     if (rc) break;
     const Unit& u = units[i];
     if (u.pass && pair_aligned(state, dtype)) {
-      rc = launch_pass(state, dtype, num_qubits, batch, gates, u, s);
+      rc = launch_pass(state, state, int64_t(1) << num_qubits, dtype, num_qubits, batch,
+                       gates + u.first, u.count, s);
     } else {
       for (int j = 0; j < u.count && rc == UA_OK; ++j)
         rc = run_gate(gates[u.first + j], u.first + j, state, state, dtype, num_qubits, batch,

@@ -154,6 +154,11 @@ bool aligned(const void* ptr, int64_t stride_elems, size_t esize, size_t align) 
 
 }  // namespace
 
+// Single 6-qubit complex64 gates on the matrix cores (blocks.hip).
+bool six_applies(int dtype, int n, int k, const void* out, const void* in, int64_t in_bstride);
+int apply_dense_six(void* out, const void* in, const void* op, int n, const int* qubits,
+                    int64_t batch, int64_t in_bstride, hipStream_t s);
+
 // Tiled dense gates on 6..kMaxWideGateK qubits (wide.hip).
 constexpr int kMaxWideGateK = 10;
 bool wide_applies(int n, int k);
@@ -254,6 +259,8 @@ int apply_operator_impl(void* out, const void* state, const void* op, int dtype,
     return check_launch("dense_fast_kernel");
   }
 
+  if (g_use_mfma && op_bstride == 0 && six_applies(dtype, n, k, out, state, state_bstride))
+    return apply_dense_six(out, state, op, n, qubits, batch, state_bstride, s);
   if (wide_applies(n, k)) {  // 6..10 targets: tiled, in place or not
     int tbc[kMaxWideGateK];
     for (int m = 0; m < k; ++m) tbc[m] = n - 1 - qubits[m];

@@ -338,6 +338,36 @@ WIDE = [(dt, n, k) for dt in (torch.complex64, torch.complex128, torch.float32)
         for n, k in [(12, 6), (13, 7), (14, 8), (14, 9), (15, 10), (16, 6)]]
 
 
+@pytest.mark.parametrize("n,sb", [(14, (2,)), (16, ()), (18, (3,))])
+def test_six_qubit_complex64_gates_on_matrix_cores(device, n, sb):
+    """One 6-qubit operator on complex64 states of at least 14 qubits runs as a
+    single-gate matrix-core pass: against the oracle out of place and in
+    place, and against the tiled vector kernel (matrix cores switched off)."""
+    from unitair_amd import _lib
+    dt = torch.complex64
+    rng = np.random.default_rng(n + len(sb))
+    qubits = [int(q) for q in rng.permutation(n)[:6]]
+    op = rand((64, 64), dt, rng) / 8
+    st = rand(sb + (2 ** n,), dt, rng)
+    opd, std = to_dev(op, dt, device), to_dev(st, dt, device)
+    out = sim.apply_operator(opd, qubits, std)
+    ref = U.apply_operator(cast_np(op, dt), qubits, cast_np(st, dt))
+    assert rel_err(out.cpu().numpy(), ref) < TOL[dt]
+    lib = _lib.load()
+    lib.ua_set_tuning(_lib.UA_TUNE_MFMA, 0)
+    try:
+        vec = sim.apply_operator(opd, qubits, std)
+    finally:
+        lib.ua_set_tuning(_lib.UA_TUNE_MFMA, 1)
+    assert rel_err(out.cpu().numpy(), vec.cpu().numpy().astype(np.complex128)) < TOL[dt]
+    x = std.clone().reshape(-1, 2 ** n)
+    rc = lib.ua_apply_operator(_lib.ptr(x), _lib.ptr(x), _lib.ptr(opd), _lib.UA_COMPLEX64, n, 6,
+                               _lib.int_array(qubits), x.shape[0], 2 ** n, 0,
+                               _lib.stream_handle(device))
+    assert rc == 0
+    assert torch.equal(x.reshape(out.shape), out)
+
+
 @pytest.mark.parametrize("dt,n,k", WIDE)
 def test_wide_gates_match_oracle_in_and_out_of_place(device, dt, n, k):
     """6..10-qubit operators on states of at least 12 qubits take the tiled
