@@ -11,6 +11,17 @@ Design (one GPU):
     `capacity` state slots (capacity x 2^n x 8 B).  A request is admitted
     when its lane has a free slot, or when a new lane fits the HBM budget;
     otherwise it waits in the queue.  Lanes are released when they empty.
+  * Admission by work, in arrival order.  The passes are bound by the vector
+    ALUs and HBM per amplitude, so running requests side by side saves
+    launches (small n) but not arithmetic: past a few million amplitudes in
+    flight, more requests only stretch every one of them (measured on 200
+    mixed requests near saturation: p99 607 ms without a cap, 456 ms with
+    2^22 or 2^24 amplitudes; the default 2^22 lets requests of 22 or more
+    qubits run alone).  A request is
+    admitted while the amplitudes of the active requests stay within
+    `max_active_amplitudes` (the first one always is), and a request that
+    does not fit holds back the ones behind it (no overtaking, so large
+    requests are not starved).
   * Continuous batching.  Each lane step advances the lane by
     K = min(max_layers_per_step, layers left of its least advanced request)
     layers, with one batched fused-pass program over the lane's occupied
@@ -261,12 +272,14 @@ class Server:
     """Continuous-batching executor on one GPU (see the module docstring).
 
     max_active limits the requests on the GPU at once (None: slots and the
-    HBM budget decide); max_slots=1, max_active=1 runs one request at a time
-    on the same fused programs."""
+    HBM budget decide); max_active_amplitudes limits the sum of 2^n over the
+    active requests (None: no limit); max_slots=1, max_active=1 runs one
+    request at a time on the same fused programs."""
 
     def __init__(self, hbm_budget: int = 64 << 30, max_slots: int = 32,
                  max_layers_per_step: int = 8, device="cuda", use_graphs: bool = True,
-                 max_active: Optional[int] = None):
+                 max_active: Optional[int] = None,
+                 max_active_amplitudes: Optional[int] = 1 << 22):
         self.budget = hbm_budget
         self.max_slots = max_slots
         self.kmax = max_layers_per_step
@@ -275,9 +288,11 @@ class Server:
             self.device = torch.device("cuda", torch.cuda.current_device())
         self.use_graphs = use_graphs
         self.max_active = max_active
+        self.max_amps = max_active_amplitudes
         self.lanes: Dict[int, _Lane] = {}
         self.used_bytes = 0
         self.active = 0
+        self.active_amps = 0
 
     def _lane_capacity(self, n: int) -> int:
         fp = (2 ** n) * BYTES_C64
@@ -287,6 +302,9 @@ class Server:
         if r.footprint > self.budget:
             raise ValueError(f"request {r.rid}: {r.footprint} B exceeds the HBM budget")
         if self.max_active is not None and self.active >= self.max_active:
+            return False
+        if self.max_amps is not None and self.active and \
+                self.active_amps + (1 << r.n) > self.max_amps:
             return False
         lane = self.lanes.get(r.n)
         if lane is None:
@@ -312,6 +330,7 @@ class Server:
             lane.states[slot, 0] = 1
         r.admitted = now
         self.active += 1
+        self.active_amps += 1 << r.n
         return True
 
     def run(self, requests: List[Request], realtime: bool = True) -> List[Request]:
@@ -330,16 +349,18 @@ class Server:
                 while i < len(pending) and pending[i].arrival <= now:
                     waiting.append(pending[i])
                     i += 1
-                still = []
-                for r in waiting:                 # arrival order
+                admitted = 0
+                for r in waiting:                 # arrival order, no overtaking
                     if not self.try_admit(r, now):
-                        still.append(r)
-                waiting = still
+                        break
+                    admitted += 1
+                waiting = waiting[admitted:]
                 progress = False
                 for lane in list(self.lanes.values()):
                     got = lane.harvest(start)
                     finished += len(got)
                     self.active -= len(got)
+                    self.active_amps -= len(got) << lane.n
                     progress = progress or bool(got)
                     while lane.active() and len(lane.inflight) < _Lane.NBUF:
                         lane.issue()
