@@ -108,3 +108,26 @@ def test_blocks_28q_roundtrip(device, dt):
     assert diff_mid > 0.1                        # the gates did something
     assert abs(norm - 1) < (1e-5 if dt == torch.complex64 else 1e-12)
     assert err < (1e-4 if dt == torch.complex64 else 1e-11)
+
+
+def test_blocks_misaligned_state_runs_gate_by_gate(device):
+    """A state that starts one complex64 element past a 16-byte boundary cannot
+    take the pair loads of a block pass: the same gate list then runs gate by
+    gate, with the same result."""
+    from unitair_amd import _lib
+    n = 16
+    circ = fused_dense_circuit(seed=12, n=n, n5=4, n4=2)
+    psi = seeded_state_np(n, (), seed=13)
+    runner = CircuitRunner(circ, dtype=torch.complex64, device=device, dense_blocks=True)
+    buf = torch.zeros(2 ** n + 1, dtype=torch.complex64, device=device)
+    view = buf[1:]
+    view.copy_(torch.from_numpy(psi).to(device, torch.complex64))
+    assert view.data_ptr() % 16 == 8
+    lib = _lib.load()
+    rc = lib.ua_run_dense_blocks(_lib.ptr(view), _lib.UA_COMPLEX64, n, 1, runner._gates,
+                                 len(circ.gates), None, _lib.stream_handle(device))
+    assert rc == 0
+    ref = oc.run_closed_form(circ.gates, psi)
+    fe, ma = BAR[torch.complex64]
+    got = view.cpu().numpy()
+    assert U.fidelity_error(ref, got) < fe and U.max_abs_error(ref, got) < ma
