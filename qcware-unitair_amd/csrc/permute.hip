@@ -115,6 +115,78 @@ __global__ __launch_bounds__(kBlock) void sign_kernel(T* out, const T* in, SignA
   }
 }
 
+// The same over adjacent pairs (i, i | 1), i even: F is linear, so
+// F(i | 1) = F(i) ^ P_0 and one table walk serves both entries, which move as
+// one 2-element access.  Needs 2-element alignment of the buffers and stride.
+template <typename T> struct PairIO;
+template <> struct PairIO<float> {
+  __device__ static void load(const float* p, float& a, float& b) {
+    const float2 w = *reinterpret_cast<const float2*>(p);
+    a = w.x; b = w.y;
+  }
+  __device__ static void store(float* p, float a, float b) {
+    *reinterpret_cast<float2*>(p) = make_float2(a, b);
+  }
+};
+template <> struct PairIO<double> {
+  __device__ static void load(const double* p, double& a, double& b) {
+    const double2 w = *reinterpret_cast<const double2*>(p);
+    a = w.x; b = w.y;
+  }
+  __device__ static void store(double* p, double a, double b) {
+    *reinterpret_cast<double2*>(p) = make_double2(a, b);
+  }
+};
+template <> struct PairIO<float2> {
+  __device__ static void load(const float2* p, float2& a, float2& b) {
+    const float4 w = *reinterpret_cast<const float4*>(p);
+    a = make_float2(w.x, w.y); b = make_float2(w.z, w.w);
+  }
+  __device__ static void store(float2* p, float2 a, float2 b) {
+    *reinterpret_cast<float4*>(p) = make_float4(a.x, a.y, b.x, b.y);
+  }
+};
+template <> struct PairIO<double2> {
+  __device__ static void load(const double2* p, double2& a, double2& b) {
+    a = make_double2(p[0].x, p[0].y); b = make_double2(p[1].x, p[1].y);
+  }
+  __device__ static void store(double2* p, double2 a, double2 b) {
+    p[0] = a; p[1] = b;
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void sign_pair_kernel(T* out, const T* in, SignArgs a) {
+  __shared__ uint64_t table[kSignChunks][256];
+  for (int c = 0; c < a.chunks; ++c) {
+    uint64_t f = 0;
+    for (int j = 0; j < 8; ++j)
+      if ((threadIdx.x >> j) & 1) f ^= a.partners[8 * c + j];
+    table[c][threadIdx.x] = f;
+  }
+  __syncthreads();
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const uint64_t imask = (uint64_t(1) << a.n) - 1;
+  const uint64_t p0 = a.partners[0];
+  for (int64_t h = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; 2 * h < a.total;
+       h += stride) {
+    const int64_t g = 2 * h;
+    const uint64_t i = uint64_t(g) & imask;  // even
+    const int64_t b = g >> a.n;
+    uint64_t f = 0;
+    for (int c = 0; c < a.chunks; ++c) f ^= table[c][(i >> (8 * c)) & 255];
+    uint32_t par0 = __popcll(i & f);
+    uint32_t par1 = __popcll((i | 1) & (f ^ p0));
+    if (a.use_all) {
+      par0 ^= (i & a.all_mask) == a.all_mask;
+      par1 ^= ((i | 1) & a.all_mask) == a.all_mask;
+    }
+    T v0, v1;
+    PairIO<T>::load(in + b * a.in_bstride + int64_t(i), v0, v1);
+    PairIO<T>::store(out + g, (par0 & 1) ? negate(v0) : v0, (par1 & 1) ? negate(v1) : v1);
+  }
+}
+
 // Controlled bit flip: out[i] = in[i ^ flip] when every bit of ctrl is set in
 // i, else in[i] (flip is not in ctrl, so the map is an involution).
 template <typename T>
@@ -247,26 +319,29 @@ extern "C" int ua_apply_sign_gates(void* out, const void* state, int dtype, int 
   a.total = batch << n;
   hipStream_t s = static_cast<hipStream_t>(stream);
   // a few elements per thread so that the table set-up is amortised
-  const int64_t want = (a.total + 16 * kBlock - 1) / (16 * kBlock);
+  const size_t es = elem_size(dtype);
+  const bool pairs = n >= 1 && reinterpret_cast<uintptr_t>(out) % (2 * es) == 0 &&
+                     reinterpret_cast<uintptr_t>(state) % (2 * es) == 0 &&
+                     state_batch_stride % 2 == 0;
+  const int64_t units = pairs ? a.total / 2 : a.total;
+  const int64_t want = (units + 16 * kBlock - 1) / (16 * kBlock);
   const unsigned grid = unsigned(want < 1 ? 1 : (want > 8192 ? 8192 : want));
+#define UA_SIGN_LAUNCH(T)                                                                   \
+  do {                                                                                      \
+    if (pairs)                                                                              \
+      sign_pair_kernel<T><<<grid, kBlock, 0, s>>>(static_cast<T*>(out),                     \
+                                                  static_cast<const T*>(state), a);         \
+    else                                                                                    \
+      sign_kernel<T><<<grid, kBlock, 0, s>>>(static_cast<T*>(out),                          \
+                                             static_cast<const T*>(state), a);              \
+  } while (0)
   switch (dtype) {
-    case UA_FLOAT32:
-      sign_kernel<float><<<grid, kBlock, 0, s>>>(static_cast<float*>(out),
-                                                 static_cast<const float*>(state), a);
-      break;
-    case UA_FLOAT64:
-      sign_kernel<double><<<grid, kBlock, 0, s>>>(static_cast<double*>(out),
-                                                  static_cast<const double*>(state), a);
-      break;
-    case UA_COMPLEX64:
-      sign_kernel<float2><<<grid, kBlock, 0, s>>>(static_cast<float2*>(out),
-                                                  static_cast<const float2*>(state), a);
-      break;
-    default:
-      sign_kernel<double2><<<grid, kBlock, 0, s>>>(static_cast<double2*>(out),
-                                                   static_cast<const double2*>(state), a);
-      break;
+    case UA_FLOAT32: UA_SIGN_LAUNCH(float); break;
+    case UA_FLOAT64: UA_SIGN_LAUNCH(double); break;
+    case UA_COMPLEX64: UA_SIGN_LAUNCH(float2); break;
+    default: UA_SIGN_LAUNCH(double2); break;
   }
+#undef UA_SIGN_LAUNCH
   return check_launch("sign_kernel");
 }
 
