@@ -174,6 +174,17 @@ def per_gate_28q(dev, seed, n=28, steps=3, warmup=1):
                     "amp_updates_per_s": len(circ.gates) * amps / sec_f,
                     "ms_per_pass": sec_f * 1e3 / prog.num_passes,
                     "pass_algorithmic_GBps": 2 * amps * 8 * prog.num_passes / sec_f / 1e9}
+    del prog
+    t0 = time.perf_counter()
+    prog = FusedProgram(circ, dtype=torch.complex64, device=dev, compiled=True)
+    compile_s = time.perf_counter() - t0
+    sec_c, _ = time_steps(prog, state, steps, warmup, with_events=False)
+    out["fused_compiled"] = {"passes": prog.num_passes, "compiled_passes": prog.num_compiled,
+                             "compile_s": compile_s, "ms_per_step": sec_c * 1e3,
+                             "amp_updates_per_s": len(circ.gates) * amps / sec_c,
+                             "ms_per_pass": sec_c * 1e3 / prog.num_passes,
+                             "pass_algorithmic_GBps":
+                                 2 * amps * 8 * prog.num_passes / sec_c / 1e9}
     del prog, state, circ
     torch.cuda.empty_cache()
     return out
@@ -719,6 +730,23 @@ def main():
         log(f"fused: {prog.num_passes} passes, {sec4 * 1e3:.3f} ms/step eager, "
             f"{sec5 * 1e3:.3f} ms/step graph")
         del prog
+        log("timing compiled fused passes")
+        t0 = time.perf_counter()
+        prog = FusedProgram(circ, dtype=torch.complex64, device=dev, compiled=True)
+        compile_s = time.perf_counter() - t0
+        sec6, _ = time_steps(prog, state, args.steps, args.warmup, with_events=False)
+        sec7 = time_graph(prog, state, args.steps, args.warmup)
+        extra["fused_compiled"] = {
+            "passes": prog.num_passes, "compiled_passes": prog.num_compiled,
+            "compile_s": compile_s, "ms_per_step": sec6 * 1e3,
+            "amp_updates_per_s": ngates * amps / sec6,
+            "graph_ms_per_step": sec7 * 1e3,
+            "graph_amp_updates_per_s": ngates * amps / sec7,
+            "ms_per_pass": sec6 * 1e3 / prog.num_passes,
+            "latency": step_latencies(prog, state, args.latency_runs)}
+        log(f"compiled fused: {prog.num_passes} passes (compiled in {compile_s:.1f} s), "
+            f"{sec6 * 1e3:.3f} ms/step eager, {sec7 * 1e3:.3f} ms/step graph")
+        del prog
 
     kernels = {k: {"launches_per_step": len(v) // args.steps,
                    "avg_us": 1e3 * float(np.mean(v))} for k, v in per_kind.items()}
@@ -770,7 +798,8 @@ def main():
         log(f"28q: per-gate {c28['ms_per_step']:.1f} ms/step "
             f"({c28['step_frac_of_hbm_peak']:.3f} of HBM peak), dense1q "
             f"{c28['dense1q_avg_us']:.0f} us; fused {c28['fused']['passes']} passes "
-            f"{c28['fused']['ms_per_step']:.2f} ms/step")
+            f"{c28['fused']['ms_per_step']:.2f} ms/step, compiled "
+            f"{c28['fused_compiled']['ms_per_step']:.2f} ms/step")
 
     if not args.no_fused_dense:
         log("fused dense 4/5-qubit blocks, 28 qubits")

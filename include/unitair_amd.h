@@ -66,8 +66,13 @@ const char* ua_last_error(void);
  *                       batch run on the matrix cores (v_mfma_f32/f64_16x16x4)
  *                       when k = 5, when k = 4 in complex128, and when k = 4
  *                       in complex64 with every target on index bits 0..5;
- *                       0: vector units for every gate. */
-enum { UA_TUNE_UNROLL = 0, UA_TUNE_LANE_EXCHANGE = 1, UA_TUNE_MFMA = 2 };
+ *                       0: vector units for every gate.
+ * UA_TUNE_COMPILED_WAVES waves per SIMD that passes compiled afterwards by
+ *                       ua_compile_fused_pass ask the register allocator for
+ *                       (complex64; 0 = default 4, else 1..8); complex128
+ *                       passes always ask for 2. */
+enum { UA_TUNE_UNROLL = 0, UA_TUNE_LANE_EXCHANGE = 1, UA_TUNE_MFMA = 2,
+       UA_TUNE_COMPILED_WAVES = 3 };
 int ua_set_tuning(int key, int value);
 
 /*
@@ -433,6 +438,35 @@ typedef struct {
 int ua_run_fused_pass(void* state, int dtype, int num_qubits, int64_t batch,
                       const ua_fused_pass* pass, const ua_fused_op* ops,
                       const ua_fused_op* host_ops, ua_stream_t stream);
+
+/*
+ * Compiled fused passes.  No counterpart in the reference (each of its gates
+ * is a pass of its own, unitair/simulation/operations.py:151-186).
+ * ua_run_fused_pass interprets a pass's op list inside one kernel;
+ * ua_compile_fused_pass writes a kernel for one pass structure (tile bits,
+ * stages, op kinds and the bits each op acts on, all constants) and compiles
+ * it with hiprtc for gfx950; ua_run_compiled_pass launches it.  Gate data is
+ * not compiled in: the kernel reads it through `ops` at run time, so a handle
+ * serves every op array with the same kinds, bits and stage split as the
+ * host_ops it was compiled from.  Handles are cached by structure and stay
+ * valid until the process ends (each GPU loads the code the first time the
+ * pass runs there).  Compilation takes of the order of a second and runs on
+ * the host only; calls on different threads compile in parallel.
+ * Requires tile_bits == 12 (UA_ERR_UNSUPPORTED otherwise); every argument
+ * check of ua_run_fused_pass applies.
+ */
+int ua_compile_fused_pass(int dtype, int num_qubits, const ua_fused_pass* pass,
+                          const ua_fused_op* host_ops, void** handle);
+/* Runs a compiled pass on `batch` contiguous states of the dtype and qubit
+ * count it was compiled for; `ops` is a device op array of the compiled
+ * structure. */
+int ua_run_compiled_pass(void* handle, void* state, int64_t batch, const ua_fused_op* ops,
+                         ua_stream_t stream);
+/* The generated source of a pass's kernel (for inspection): *length gets its
+ * size in bytes; up to capacity - 1 bytes and a terminating NUL go to buf. */
+int ua_fused_pass_source(int dtype, int num_qubits, const ua_fused_pass* pass,
+                         const ua_fused_op* host_ops, char* buf, int64_t capacity,
+                         int64_t* length);
 
 /*
  * Differentiation.  unitair relies on torch autograd through its op sequence

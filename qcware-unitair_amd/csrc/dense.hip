@@ -24,6 +24,7 @@
 //   SCALAR: fallback for misaligned views.
 // For k >= 3 the (permuted) matrix is staged in LDS once per workgroup.
 #include "dense_kernels.h"
+#include "fused_common.h"
 
 namespace ua {
 
@@ -319,6 +320,8 @@ extern "C" int ua_set_tuning(int key, int value) {
     case UA_TUNE_MFMA:
       ua::g_use_mfma = value != 0;
       return UA_OK;
+    case UA_TUNE_COMPILED_WAVES:
+      return ua::set_compiled_waves(value);
   }
   ua::set_error("unknown tuning key %d", key);
   return UA_ERR_INVALID_ARGUMENT;

@@ -27,6 +27,7 @@
 // LDS addresses of a cube are one per-thread base XOR a wave-uniform offset
 // (the swizzle is linear over GF(2)), one VALU op per element.
 #include "ua_common.h"
+#include "fused_common.h"
 
 namespace ua {
 namespace {
@@ -706,15 +707,15 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd<T>()))) void fused_pass_kernel
   store(t_end);
 }
 
+}  // namespace
+
 bool is_diag_kind(int kind) {
   return kind == UA_FOP_CZ || kind == UA_FOP_DIAG1 || kind == UA_FOP_PHASE ||
          kind == UA_FOP_CZ_LAYER || kind == UA_FOP_PHASE_FACTORS;
 }
 
-}  // namespace
-
-int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused_pass* pass,
-                    const ua_fused_op* ops, const ua_fused_op* host_ops, hipStream_t s) {
+int check_fused_pass(int dtype, int n, const ua_fused_pass* pass, const ua_fused_op* host_ops,
+                     FusedPassInfo* info) {
   UA_CHECK_ARG(dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128,
                "fused passes run on complex64/complex128 states");
   UA_CHECK_ARG(pass != nullptr, "NULL pass");
@@ -723,30 +724,19 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
   UA_CHECK_ARG(m >= 4 && m <= kTileBits && m <= n, "tile_bits=%d invalid for %d qubits", m, n);
   UA_CHECK_ARG(pass->num_stages >= 0 && pass->num_stages <= UA_FUSED_MAX_STAGES,
                "num_stages=%d out of range", pass->num_stages);
-  UA_CHECK_ARG(batch >= 0, "negative batch");
-  if (batch == 0) return UA_OK;
-  UA_CHECK_ARG(state && (pass->num_stages == 0 || ops), "NULL pointer");
   UA_CHECK_ARG(host_ops != nullptr || pass->num_stages == 0,
                "host_ops (the host copy of ops) is required");
-  UA_CHECK_ARG(reinterpret_cast<uintptr_t>(state) % (2 * elem_size(dtype)) == 0,
-               "state must be aligned to two elements");
   UA_CHECK_ARG(pass->stage_ops[0] == 0, "stage_ops[0] must be 0");
   const size_t rsize = elem_size(dtype) / 2;
-  PassArgs a{};
-  a.n = n;
-  a.m = m;
-  a.log2_tiles_per_state = n - m;
-  a.total_tiles = batch << (n - m);
-  a.num_stages = pass->num_stages;
   for (int i = 0; i < m; ++i) {
-    a.local[i] = pass->local_bits[i];
-    UA_CHECK_ARG(a.local[i] >= 0 && a.local[i] < n, "local bit %d out of range", a.local[i]);
-    UA_CHECK_ARG(i >= 4 ? a.local[i] > a.local[i - 1] : a.local[i] == i,
+    const int l = pass->local_bits[i];
+    UA_CHECK_ARG(l >= 0 && l < n, "local bit %d out of range", l);
+    UA_CHECK_ARG(i >= 4 ? l > pass->local_bits[i - 1] : l == i,
                  "local bits must be ascending and start 0,1,2,3");
   }
   bool heavy = false, only_layers = true, needs_index = false;
   bool stage_not_factors[UA_FUSED_MAX_STAGES] = {};
-  for (int st = 0; st < a.num_stages; ++st) {
+  for (int st = 0; st < pass->num_stages; ++st) {
     UA_CHECK_ARG(pass->stage_ops[st + 1] >= pass->stage_ops[st], "stage %d: bad op range", st);
     const bool diag = pass->stage_bits[st][0] < 0;
     if (diag) {
@@ -764,8 +754,6 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
         seen |= 1u << t;
       }
     }
-    for (int k = 0; k < 4; ++k) a.stage_bits[st][k] = pass->stage_bits[st][k];
-    a.stage_ops[st] = pass->stage_ops[st];
     for (int o = pass->stage_ops[st]; o < pass->stage_ops[st + 1]; ++o) {
       const ua_fused_op& op = host_ops[o];
       UA_CHECK_ARG(op.kind >= UA_FOP_DENSE1 && op.kind <= UA_FOP_PHASE_FACTORS,
@@ -807,7 +795,8 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
         UA_CHECK_ARG(op.a >= 0 && op.a < 4 && op.b >= 0 && op.b < 4 && op.a != op.b,
                      "op %d: bad cube bits", o);
       if (op.kind == UA_FOP_CZ)
-        UA_CHECK_ARG(op.a >= 0 && op.a < n && op.b >= 0 && op.b < n, "op %d: bad bits", o);
+        UA_CHECK_ARG(op.a >= 0 && op.a < n && op.b >= 0 && op.b < n && op.a != op.b,
+                     "op %d: bad bits", o);
       if (op.kind == UA_FOP_DIAG1) UA_CHECK_ARG(op.a >= 0 && op.a < n, "op %d: bad bit", o);
       UA_CHECK_ARG(op.data_bstride >= 0, "op %d: negative data_bstride", o);
       UA_CHECK_ARG(op.data || op.kind == UA_FOP_CZ || op.kind == UA_FOP_CNOT,
@@ -822,11 +811,42 @@ int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused
                      "op %d: phase factors must be aligned to two elements", o);
     }
   }
+  const int ns = pass->num_stages;
+  info->heavy = heavy;
+  info->only_layers = only_layers;
+  info->needs_index = needs_index;
+  info->reg_head = ns > 0 && pass->stage_bits[0][0] < 0 && !stage_not_factors[0];
+  info->reg_tail = ns > 0 && pass->stage_bits[ns - 1][0] < 0 && !stage_not_factors[ns - 1];
+  return UA_OK;
+}
+
+int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused_pass* pass,
+                    const ua_fused_op* ops, const ua_fused_op* host_ops, hipStream_t s) {
+  FusedPassInfo info{};
+  const int rc = check_fused_pass(dtype, n, pass, host_ops, &info);
+  if (rc != UA_OK) return rc;
+  UA_CHECK_ARG(batch >= 0, "negative batch");
+  if (batch == 0) return UA_OK;
+  UA_CHECK_ARG(state && (pass->num_stages == 0 || ops), "NULL pointer");
+  UA_CHECK_ARG(reinterpret_cast<uintptr_t>(state) % (2 * elem_size(dtype)) == 0,
+               "state must be aligned to two elements");
+  const int m = pass->tile_bits;
+  PassArgs a{};
+  a.n = n;
+  a.m = m;
+  a.log2_tiles_per_state = n - m;
+  a.total_tiles = batch << (n - m);
+  a.num_stages = pass->num_stages;
+  for (int i = 0; i < m; ++i) a.local[i] = pass->local_bits[i];
+  for (int st = 0; st < a.num_stages; ++st) {
+    for (int k = 0; k < 4; ++k) a.stage_bits[st][k] = pass->stage_bits[st][k];
+    a.stage_ops[st] = pass->stage_ops[st];
+  }
   a.stage_ops[a.num_stages] = pass->stage_ops[a.num_stages];
-  a.needs_index = needs_index ? 1 : 0;
-  const int ns = a.num_stages;
-  a.reg_head = ns > 0 && a.stage_bits[0][0] < 0 && !stage_not_factors[0];
-  a.reg_tail = ns > 0 && a.stage_bits[ns - 1][0] < 0 && !stage_not_factors[ns - 1];
+  a.needs_index = info.needs_index ? 1 : 0;
+  a.reg_head = info.reg_head ? 1 : 0;
+  a.reg_tail = info.reg_tail ? 1 : 0;
+  const bool heavy = info.heavy, only_layers = info.only_layers;
   const int64_t tiles_per_block = int64_t(kTile) >> m;
   const int64_t blocks = (a.total_tiles + tiles_per_block - 1) / tiles_per_block;
   UA_CHECK_ARG(blocks <= 0x7fffffffLL, "too many tiles");

@@ -91,7 +91,7 @@ def build(verbose: bool = False, changed_only: bool = False, asan: bool = False)
 def _link(objs, path: Path, extra=()):
     tmp = path.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *extra,
-           *map(str, objs), "-o", str(tmp)]
+           *map(str, objs), "-lhiprtc", "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -81,6 +81,7 @@ SIGNATURES = {
 UA_TUNE_UNROLL = 0
 UA_TUNE_LANE_EXCHANGE = 1
 UA_TUNE_MFMA = 2
+UA_TUNE_COMPILED_WAVES = 3
 
 UA_PAULI_X = 0
 UA_PAULI_Y = 1
@@ -124,6 +125,11 @@ SIGNATURES["ua_run_fused_pass"] = (_i32, [_p, _i32, _i32, _i64, ctypes.POINTER(F
 SIGNATURES["ua_run_circuit"] = (_i32, [_p, _p, _i32, _i32, _i64, ctypes.POINTER(Gate), _i32,
                                        ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.POINTER(ctypes.c_int), _p])
+SIGNATURES["ua_compile_fused_pass"] = (_i32, [_i32, _i32, ctypes.POINTER(FusedPass), _p,
+                                              ctypes.POINTER(ctypes.c_void_p)])
+SIGNATURES["ua_run_compiled_pass"] = (_i32, [_p, _p, _i64, _p, _p])
+SIGNATURES["ua_fused_pass_source"] = (_i32, [_i32, _i32, ctypes.POINTER(FusedPass), _p,
+                                             ctypes.c_char_p, _i64, _pi64])
 UA_FUSE_MAX_OPS = 64
 SIGNATURES["ua_fuse_1q_ops"] = (_i32, [_p, ctypes.POINTER(ctypes.c_void_p), _i32, _i32,
                                        ctypes.POINTER(ctypes.c_int), _i32, _i64, _i64, _i32, _p])

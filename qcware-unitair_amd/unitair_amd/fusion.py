@@ -16,7 +16,9 @@ the host first (complex128), as unitair's fuse_single_qubit_operators
 """
 from __future__ import annotations
 
+import concurrent.futures
 import ctypes
+import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -471,6 +473,87 @@ def _stages(p: _Pass, reorder: bool = True):
     return result
 
 
+def _pass_structs(items, n: int, tile_bits: int = TILE_BITS, reorder: bool = True):
+    """The plan of `items` as kernel structures, without touching a device:
+    steps ("fused", FusedPass, first op, op count) | ("wide", qubits, item),
+    the flat FusedOp list (data pointers unset) and the (op index, item) pairs
+    whose data the ops read."""
+    plan = plan_passes(items, n, tile_bits, reorder)
+    steps, op_structs = [], []
+    pending = []             # (op index, item) whose data pointer is set later
+    for p in plan:
+        if isinstance(p, _Item):
+            qubits = [n - 1 - b for b in p.bits]
+            steps.append(("wide", qubits, p))
+            continue
+        stages = _stages(p, reorder)
+        for chunk in range(0, len(stages), MAX_STAGES):
+            part = stages[chunk:chunk + MAX_STAGES]
+            fp = FusedPass()
+            fp.tile_bits = len(p.local)
+            for i, s in enumerate(p.local):
+                fp.local_bits[i] = s
+            fp.num_stages = len(part)
+            first_op = len(op_structs)
+            needs_index = 0
+            tile_of = {s: i for i, s in enumerate(p.local)}
+            for si, (cube, ops) in enumerate(part):
+                for k in range(CUBE_BITS):
+                    fp.stage_bits[si][k] = cube[k] if cube is not None else -1
+                fp.stage_ops[si] = len(op_structs) - first_op
+                if cube is None:
+                    pos = {}
+                    entries = [("op", it) for it in ops]
+                else:
+                    pos = {t: i for i, t in enumerate(cube)}
+                    entries = _segment_stage(ops, [p.local[t] for t in cube])
+                flat_ops = []
+                for kind, x in entries:
+                    if kind == "op":
+                        flat_ops.append(x)
+                        continue
+                    head = FusedOp()
+                    head.kind = FOP_LAYER1Q
+                    head.a = sum(len(lane) << (8 * k) for k, lane in enumerate(x))
+                    flat_ops.append(head)
+                    for lane in x:
+                        flat_ops.extend(lane)
+                for it in flat_ops:
+                    if isinstance(it, FusedOp):       # a layer head
+                        op_structs.append(it)
+                        continue
+                    o = FusedOp()
+                    if it.kind == "dense1":
+                        o.kind, o.a = FOP_DENSE1, pos[tile_of[it.bits[0]]]
+                    elif it.kind == "dense2":
+                        o.kind = FOP_DENSE2
+                        o.a, o.b = pos[tile_of[it.bits[0]]], pos[tile_of[it.bits[1]]]
+                    elif it.kind == "cnot":
+                        o.kind, o.a, o.b = FOP_CNOT, it.bits[0], pos[tile_of[it.bits[1]]]
+                        needs_index = 1
+                    elif it.kind == "cz":
+                        o.kind, o.a, o.b = FOP_CZ, it.bits[0], it.bits[1]
+                        needs_index = 1
+                    elif it.kind == "diag1":
+                        o.kind, o.a = FOP_DIAG1, it.bits[0]
+                        needs_index = 1
+                    elif it.kind == "phase":
+                        o.kind = FOP_PHASE_FACTORS
+                        needs_index = 1
+                    elif it.kind == "cz_layer":
+                        o.kind = FOP_CZ_LAYER
+                        needs_index = 1
+                    else:
+                        raise ValueError(it.kind)
+                    if it.data is not None:
+                        pending.append((len(op_structs), it))
+                    op_structs.append(o)
+            fp.stage_ops[len(part)] = len(op_structs) - first_op
+            fp.needs_index = needs_index
+            steps.append(("fused", fp, first_op, len(op_structs) - first_op))
+    return steps, op_structs, pending
+
+
 class FusedProgram:
     """A circuit compiled into fused passes for one (n, dtype, device).
 
@@ -479,23 +562,26 @@ class FusedProgram:
 
     def __init__(self, circuit: Circuit, dtype: torch.dtype = torch.complex64,
                  device="cuda", tile_bits: int = TILE_BITS, merge_1q: bool = True,
-                 reorder: bool = True):
+                 reorder: bool = True, compiled: bool = False, phase_angles: bool = False):
         items = _items_from_circuit(circuit)
         if merge_1q:
             items = _merge_1q(items)
-        self._build(items, circuit.num_qubits, dtype, device, tile_bits, reorder)
+        self._build(items, circuit.num_qubits, dtype, device, tile_bits, reorder, compiled,
+                    phase_angles)
         self.num_gates = len(circuit.gates)
 
     @classmethod
     def from_items(cls, items: List[_Item], n: int, dtype: torch.dtype = torch.complex64,
                    device="cuda", tile_bits: int = TILE_BITS,
-                   reorder: bool = True) -> "FusedProgram":
+                   reorder: bool = True, compiled: bool = False,
+                   phase_angles: bool = False) -> "FusedProgram":
         prog = cls.__new__(cls)
-        prog._build(list(items), n, dtype, device, tile_bits, reorder)
+        prog._build(list(items), n, dtype, device, tile_bits, reorder, compiled, phase_angles)
         prog.num_gates = len(items)
         return prog
 
-    def _build(self, items, n, dtype, device, tile_bits, reorder=True):
+    def _build(self, items, n, dtype, device, tile_bits, reorder=True, compiled=False,
+               phase_angles=False):
         if dtype not in (torch.complex64, torch.complex128):
             raise ValueError("fused programs run complex64/complex128 states")
         for it in items:
@@ -511,83 +597,16 @@ class FusedProgram:
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         real = torch.float32 if dtype == torch.complex64 else torch.float64
-        plan = plan_passes(items, self.n, tile_bits, reorder)
+        steps, op_structs, pending = _pass_structs(items, self.n, tile_bits, reorder)
         self._keep = []
-        self.steps = []          # ("fused", FusedPass, ops_ptr, nops) | ("wide", qubits, mat)
-        mats, op_structs = [], []
-        pending = []             # (op index, item) whose data pointer is set later
-        for p in plan:
-            if isinstance(p, _Item):
-                m = torch.from_numpy(p.data).to(self.device, dtype)
+        self.steps = []
+        for st in steps:
+            if st[0] == "wide":
+                m = torch.from_numpy(st[2].data).to(self.device, dtype)
                 self._keep.append(m)
-                qubits = [self.n - 1 - b for b in p.bits]
-                self.steps.append(("wide", qubits, m))
-                continue
-            stages = _stages(p, reorder)
-            for chunk in range(0, len(stages), MAX_STAGES):
-                part = stages[chunk:chunk + MAX_STAGES]
-                fp = FusedPass()
-                fp.tile_bits = len(p.local)
-                for i, s in enumerate(p.local):
-                    fp.local_bits[i] = s
-                fp.num_stages = len(part)
-                first_op = len(op_structs)
-                needs_index = 0
-                tile_of = {s: i for i, s in enumerate(p.local)}
-                for si, (cube, ops) in enumerate(part):
-                    for k in range(CUBE_BITS):
-                        fp.stage_bits[si][k] = cube[k] if cube is not None else -1
-                    fp.stage_ops[si] = len(op_structs) - first_op
-                    if cube is None:
-                        pos = {}
-                        entries = [("op", it) for it in ops]
-                    else:
-                        pos = {t: i for i, t in enumerate(cube)}
-                        entries = _segment_stage(ops, [p.local[t] for t in cube])
-                    flat_ops = []
-                    for kind, x in entries:
-                        if kind == "op":
-                            flat_ops.append(x)
-                            continue
-                        head = FusedOp()
-                        head.kind = FOP_LAYER1Q
-                        head.a = sum(len(lane) << (8 * k) for k, lane in enumerate(x))
-                        flat_ops.append(head)
-                        for lane in x:
-                            flat_ops.extend(lane)
-                    for it in flat_ops:
-                        if isinstance(it, FusedOp):       # a layer head
-                            op_structs.append(it)
-                            continue
-                        o = FusedOp()
-                        if it.kind == "dense1":
-                            o.kind, o.a = FOP_DENSE1, pos[tile_of[it.bits[0]]]
-                        elif it.kind == "dense2":
-                            o.kind = FOP_DENSE2
-                            o.a, o.b = pos[tile_of[it.bits[0]]], pos[tile_of[it.bits[1]]]
-                        elif it.kind == "cnot":
-                            o.kind, o.a, o.b = FOP_CNOT, it.bits[0], pos[tile_of[it.bits[1]]]
-                            needs_index = 1
-                        elif it.kind == "cz":
-                            o.kind, o.a, o.b = FOP_CZ, it.bits[0], it.bits[1]
-                            needs_index = 1
-                        elif it.kind == "diag1":
-                            o.kind, o.a = FOP_DIAG1, it.bits[0]
-                            needs_index = 1
-                        elif it.kind == "phase":
-                            o.kind = FOP_PHASE_FACTORS
-                            needs_index = 1
-                        elif it.kind == "cz_layer":
-                            o.kind = FOP_CZ_LAYER
-                            needs_index = 1
-                        else:
-                            raise ValueError(it.kind)
-                        if it.data is not None:
-                            pending.append((len(op_structs), it))
-                        op_structs.append(o)
-                fp.stage_ops[len(part)] = len(op_structs) - first_op
-                fp.needs_index = needs_index
-                self.steps.append(("fused", fp, first_op, len(op_structs) - first_op))
+                self.steps.append(("wide", st[1], m))
+            else:
+                self.steps.append(st)
         for o in op_structs:
             o.cache_slot = -1           # unused by the kernel (gate data is read as scalars)
         # upload operand data and fill data pointers
@@ -603,7 +622,10 @@ class FusedProgram:
                 t = it.data.to(self.device, want).contiguous()
             else:
                 t = torch.as_tensor(np.ascontiguousarray(it.data)).to(self.device, want)
-            if it.kind == "phase":
+            if it.kind == "phase" and phase_angles:
+                # the pass reads the angles and forms exp(-1j * angle) itself
+                o.kind = FOP_PHASE
+            elif it.kind == "phase":
                 # exp(-1j * angles), formed once by the drop-in phase kernel
                 # (the same factors apply_phase multiplies by); the pass
                 # then multiplies by them
@@ -630,6 +652,29 @@ class FusedProgram:
                 _, qubits, m = st
                 self._calls.append(("wide", _lib.int_array(qubits), len(qubits), m))
         self.num_passes = len(self._calls)
+        self.num_compiled = 0
+        if compiled:
+            self._compile()
+
+    def _compile(self):
+        """Replace every pass with 12 tile bits by its compiled kernel
+        (ua_compile_fused_pass; the compiles run on parallel threads)."""
+        todo = [i for i, c in enumerate(self._calls) if c[0] == "fused" and c[1].tile_bits == 12]
+
+        def one(i):
+            fp, host = self._calls[i][1], self._calls[i][3]
+            handle = ctypes.c_void_p()
+            rc = self.lib.ua_compile_fused_pass(self.code, self.n, ctypes.byref(fp),
+                                                ctypes.c_void_p(host), ctypes.byref(handle))
+            return rc, handle
+
+        workers = max(1, min(8, len(todo), os.cpu_count() or 1))
+        with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+            results = list(ex.map(one, todo))
+        for i, (rc, handle) in zip(todo, results):
+            _lib.check(rc, "compiling a fused pass")
+            self._calls[i] = ("compiled", handle, self._calls[i][2])
+        self.num_compiled = len(todo)
 
     def run(self, state: torch.Tensor) -> torch.Tensor:
         """Apply the program to a contiguous (batch, 2**n) state in place."""
@@ -640,7 +685,10 @@ class FusedProgram:
         batch = state.numel() // (2 ** self.n)
         stream = _lib.stream_handle(self.device)
         for c in self._calls:
-            if c[0] == "fused":
+            if c[0] == "compiled":
+                rc = self.lib.ua_run_compiled_pass(c[1], _lib.ptr(state), batch,
+                                                   ctypes.c_void_p(c[2]), stream)
+            elif c[0] == "fused":
                 rc = self.lib.ua_run_fused_pass(_lib.ptr(state), self.code, self.n, batch,
                                                 ctypes.byref(c[1]), ctypes.c_void_p(c[2]),
                                                 ctypes.c_void_p(c[3]), stream)

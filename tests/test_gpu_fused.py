@@ -4,6 +4,7 @@ import pytest
 import torch
 
 from unitair_amd.fusion import FusedProgram
+from unitair_amd.gates import haar_unitary
 from unitair_amd.initializations import seeded_state_np
 from oracle import circuits as oc
 from oracle import unitair_np as U
@@ -64,6 +65,97 @@ def test_fused_graph_replay_matches_eager(device):
     prog.capture(state)
     for _ in range(3):
         state.copy_(psi)                 # eager work between replays
+        torch.cuda.synchronize()
+        prog.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(state, eager)
+
+
+# ---- compiled passes (ua_compile_fused_pass / ua_run_compiled_pass): the same
+# circuits, every pass with 12 tile bits run by its own compiled kernel
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("n,batch", [(12, 3), (13, 1), (16, 2), (20, 1)])
+def test_compiled_random_circuits(device, dt, n, batch):
+    circ = random_mixed_circuit(n, 80, seed=n * 7 + batch)
+    prog = FusedProgram(circ, dtype=dt, device=device, compiled=True)
+    assert prog.num_compiled > 0
+    psi = seeded_state_np(n, (batch,), seed=n)
+    ref = np.stack([oc.run_closed_form(circ.gates, p) for p in psi])
+    out = prog.run(torch.from_numpy(psi).to(device, dt)).cpu().numpy()
+    assert rel_err(out, ref) < TOL[dt] * 10, (prog.num_passes, n)
+    assert np.max(U.fidelity_error(ref, out)) < TOL[dt]
+
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
+def test_compiled_random_circuit_24q(device, dt):
+    from test_gpu_parity import _circuit_24q_and_oracle
+    circ, psi, ref = _circuit_24q_and_oracle()
+    prog = FusedProgram(circ, dtype=dt, device=device, compiled=True)
+    assert prog.num_compiled == prog.num_passes
+    out = prog.run(torch.from_numpy(psi).to(device, dt)).cpu().numpy()
+    fid = float(U.fidelity_error(ref, out))
+    mae = U.max_abs_error(ref, out)
+    print(f"compiled 24q {dt}: {prog.num_passes} passes, fidelity error {fid:.3e}, "
+          f"max-abs {mae:.3e}")
+    bar = {torch.complex64: (1e-10, 1e-7), torch.complex128: (1e-14, 1e-14)}[dt]
+    assert fid < bar[0] and mae < bar[1], (fid, mae)
+
+
+def _per_state_items(n, batch, seed, device, dt):
+    """Gate items of a random mixed circuit plus per-state 1-qubit matrices,
+    diagonals and CZ-layer masks (the forms the adjoint and serving loops
+    build), on state bits inside and outside the tile."""
+    from unitair_amd.fusion import _Item, _items_from_circuit, _merge_1q
+    rng = np.random.default_rng(seed)
+    items = _merge_1q(_items_from_circuit(random_mixed_circuit(n, 50, seed=seed)))
+    items = [it for it in items if it.kind != "wide"]
+    for _ in range(40):
+        r = rng.random()
+        b = int(rng.integers(n))
+        if r < 0.4:
+            m = np.stack([haar_unitary(2, rng) for _ in range(batch)])
+            items.append(_Item("dense1", (b,), torch.from_numpy(m).to(device, dt), True))
+        elif r < 0.7:
+            d = np.exp(1j * rng.uniform(0, 2 * np.pi, size=(batch, 2)))
+            items.append(_Item("diag1", (b,), torch.from_numpy(d).to(device, dt), True))
+        elif r < 0.85:
+            masks = rng.integers(0, 2 ** (n - 1), size=batch, dtype=np.int64)
+            items.append(_Item("cz_layer", (), torch.from_numpy(masks).to(device), True))
+        else:
+            a, c = rng.choice(n, 2, replace=False)
+            items.append(_Item("cnot" if r < 0.93 else "cz", (int(a), int(c))))
+    return items
+
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("n,batch,reorder", [(12, 4, True), (14, 3, True), (14, 2, False),
+                                             (17, 2, True)])
+def test_compiled_matches_interpreter_per_state_data(device, dt, n, batch, reorder):
+    """Compiled passes against the interpreter (pinned against the oracle by
+    the tests above and by the serving and adjoint tests) on the op kinds and
+    per-state data those loops use."""
+    items = _per_state_items(n, batch, seed=n * 3 + batch, device=device, dt=dt)
+    interp = FusedProgram.from_items(items, n, dt, device, reorder=reorder)
+    comp = FusedProgram.from_items(items, n, dt, device, reorder=reorder, compiled=True)
+    assert comp.num_compiled == comp.num_passes
+    psi = torch.from_numpy(seeded_state_np(n, (batch,), seed=5)).to(device, dt)
+    a = interp.run(psi.clone()).cpu().numpy()
+    c = comp.run(psi.clone()).cpu().numpy()
+    assert rel_err(c, a) < {torch.complex64: 1e-5, torch.complex128: 1e-13}[dt]
+
+
+def test_compiled_graph_replay_matches_eager(device):
+    n = 14
+    circ = random_mixed_circuit(n, 60, seed=5)
+    prog = FusedProgram(circ, dtype=torch.complex64, device=device, compiled=True)
+    assert prog.num_compiled > 0
+    psi = torch.from_numpy(seeded_state_np(n, (2,), seed=3)).to(device, torch.complex64)
+    eager = prog.run(psi.clone())
+    state = psi.clone()
+    prog.capture(state)
+    for _ in range(3):
+        state.copy_(psi)
         torch.cuda.synchronize()
         prog.replay()
         torch.cuda.synchronize()
