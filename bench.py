@@ -270,6 +270,25 @@ def adjoint_bench(dev, iters=3):
         adj_step(th, ph)
     torch.cuda.synchronize()
     adj_s = (time.perf_counter() - t0) / iters
+    moved = adjoint.last_traffic["forward_bytes"] + adjoint.last_traffic["backward_bytes"]
+
+    def adj_step_compiled(t, p):
+        t = t.detach().requires_grad_(True)
+        p = p.detach().requires_grad_(True)
+        adjoint.sum_z_loss(t, p, n, dtype=torch.complex64, compiled=True).sum().backward()
+        return t.grad, p.grad
+
+    t0 = time.perf_counter()
+    g_ref = adj_step(th, ph)
+    g_c = adj_step_compiled(th, ph)         # the first call compiles the passes
+    torch.cuda.synchronize()
+    first_compiled_s = time.perf_counter() - t0
+    compiled_vs_interp = max(float((a - b).abs().max()) for a, b in zip(g_ref, g_c))
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        adj_step_compiled(th, ph)
+    torch.cuda.synchronize()
+    adj_c_s = (time.perf_counter() - t0) / iters
 
     Bp = 64
     pop = torch.tensor([bin(v).count("1") for v in range(2 ** n)], device=dev,
@@ -303,7 +322,6 @@ def adjoint_bench(dev, iters=3):
     plain_step(th[:Bp], ph[:Bp])
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
-    moved = adjoint.last_traffic["forward_bytes"] + adjoint.last_traffic["backward_bytes"]
     return {"workload": f"{n} qubits, {L} layers (RZ.RY on every qubit + CZ ring), loss "
                         f"sum_i <Z_i>, complex64, forward + gradient of all {2 * L * n} "
                         f"angles per state",
@@ -312,6 +330,11 @@ def adjoint_bench(dev, iters=3):
             "adjoint_algorithmic_bytes": moved,
             "adjoint_algorithmic_GBps": moved / adj_s / 1e9,
             "adjoint_frac_of_hbm_peak": moved / adj_s / 1e9 / HBM_PEAK_GBS,
+            "compiled_passes": {"adjoint_ms": adj_c_s * 1e3,
+                                "adjoint_states_per_s": B / adj_c_s,
+                                "adjoint_algorithmic_GBps": moved / adj_c_s / 1e9,
+                                "first_call_with_compile_s": first_compiled_s,
+                                "max_grad_difference_vs_interpreted": compiled_vs_interp},
             "plain_per_gate_autograd_batch": Bp, "plain_per_gate_autograd_ms": plain_s * 1e3,
             "plain_per_gate_autograd_states_per_s": Bp / plain_s,
             "speedup_states_per_s": (B / adj_s) / (Bp / plain_s)}
@@ -772,7 +795,8 @@ def main():
     if not args.no_adjoint:
         log("adjoint gradient workload")
         extra["adjoint"] = adjoint_bench(dev)
-        log(f"adjoint: {extra['adjoint']['adjoint_ms']:.1f} ms for 1024 states; plain "
+        log(f"adjoint: {extra['adjoint']['adjoint_ms']:.1f} ms for 1024 states (compiled "
+            f"passes {extra['adjoint']['compiled_passes']['adjoint_ms']:.1f} ms); plain "
             f"per-gate autograd {extra['adjoint']['plain_per_gate_autograd_ms']:.1f} ms for 64")
 
     if not args.no_serving:

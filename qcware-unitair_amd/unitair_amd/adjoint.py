@@ -56,6 +56,7 @@ class _Spec:
 
     def __init__(self, gates: Sequence[tuple], n: int, num_params: int):
         self.n = n
+        self.compiled = False
         segments, fixed, rot = [], [], []
         for g in gates:
             kind = g[0]
@@ -140,8 +141,8 @@ def _rot_items(s, rot, table, n):
             for r, (kind, q, idx, coef) in enumerate(rot)]
 
 
-def _program(items, n, dtype, device):
-    return FusedProgram.from_items(items, n, dtype, device) if items else None
+def _program(items, n, dtype, device, compiled=False):
+    return FusedProgram.from_items(items, n, dtype, device, compiled=compiled) if items else None
 
 
 def _check_inputs(params, observable, n, dtype, device):
@@ -175,18 +176,18 @@ class _Programs:
             for s in range(c0, min(S, c0 + self.every)):
                 fixed, rot = spec.segments[s]
                 items += list(fixed) + _rot_items(s, rot, fwd_table, n)
-            self.fwd.append((c0, _program(items, n, dtype, dev)))
-        self.trail = _program(list(spec.trailing), n, dtype, dev)
+            self.fwd.append((c0, _program(items, n, dtype, dev, spec.compiled)))
+        self.trail = _program(list(spec.trailing), n, dtype, dev, spec.compiled)
         self.inv_trail = self.inv = None
         if backward:
             inv_table = _rotation_table(spec, p, dtype, -1.0, repeat=2)
             self.inv_trail = _program([_inverse_fixed(it) for it in reversed(spec.trailing)],
-                                      n, dtype, dev)
+                                      n, dtype, dev, spec.compiled)
             self.inv = []
             for s, (fixed, rot) in enumerate(spec.segments):
                 items = _rot_items(s, rot, inv_table, n)
                 items += [_inverse_fixed(it) for it in reversed(fixed)]
-                self.inv.append(_program(items, n, dtype, dev))
+                self.inv.append(_program(items, n, dtype, dev, spec.compiled))
 
 
 # Algorithmic HBM bytes of the last forward and backward (state passes, copies,
@@ -304,23 +305,29 @@ def _diag_loss(obs: torch.Tensor, psi: torch.Tensor) -> torch.Tensor:
 
 def expectation(params: torch.Tensor, gates: Sequence[tuple], observable: torch.Tensor,
                 num_qubits: int, dtype: torch.dtype = torch.complex64,
-                initial_state: Optional[torch.Tensor] = None) -> torch.Tensor:
+                initial_state: Optional[torch.Tensor] = None,
+                compiled: bool = False) -> torch.Tensor:
     """loss[b] = diag_expectation_value(observable, U(params[b]) |psi0>) in
     float64, differentiable in params (adjoint backward) and in the
     observable.  params: (batch, num_params) real; gates: see the module
     docstring; observable: (2^n,) or (batch, 2^n) real; initial_state:
-    |0...0> by default, else (2^n,) or (batch, 2^n) (not differentiated)."""
+    |0...0> by default, else (2^n,) or (batch, 2^n) (not differentiated).
+    compiled: run the passes with 12 tile bits (n >= 12) as compiled kernels
+    (FusedProgram(compiled=True)); the first call with a gate list compiles
+    them, later calls with the same gate list reuse them."""
     n = int(num_qubits)
     if not isinstance(observable, torch.Tensor):
         observable = torch.as_tensor(observable, device=params.device)
     _check_inputs(params, observable, n, dtype, params.device)
     spec = _Spec(gates, n, params.shape[1])
+    spec.compiled = compiled
     return _AdjointExpectation.apply(params, observable, spec, dtype, initial_state)
 
 
 def run_gates(params: torch.Tensor, gates: Sequence[tuple], num_qubits: int,
               dtype: torch.dtype = torch.complex64,
-              initial_state: Optional[torch.Tensor] = None) -> torch.Tensor:
+              initial_state: Optional[torch.Tensor] = None,
+              compiled: bool = False) -> torch.Tensor:
     """Final states (batch, 2^n) of the circuit for every parameter row."""
     n = int(num_qubits)
     obs = torch.zeros(2 ** n, dtype=torch.float64, device=params.device)
@@ -334,7 +341,7 @@ def run_gates(params: torch.Tensor, gates: Sequence[tuple], num_qubits: int,
         for s, (fixed, rot) in enumerate(spec.segments):
             items += list(fixed) + _rot_items(s, rot, table, n)
         items += list(spec.trailing)
-        prog = _program(items, n, dtype, dev)
+        prog = _program(items, n, dtype, dev, compiled)
         psi = _initial(B, n, dtype, dev, initial_state)
         if prog is not None:
             prog.run(psi)
@@ -381,19 +388,20 @@ def _ansatz_params(theta, phi):
 
 
 def run_circuit(theta: torch.Tensor, phi: torch.Tensor, num_qubits: int,
-                dtype: torch.dtype = torch.complex64) -> torch.Tensor:
+                dtype: torch.dtype = torch.complex64, compiled: bool = False) -> torch.Tensor:
     """Final states (batch, 2^n) of the hardware-efficient circuit."""
     _check(theta, phi, num_qubits)
     return run_gates(_ansatz_params(theta, phi), hardware_efficient_gates(num_qubits,
                                                                           theta.shape[1]),
-                     num_qubits, dtype)
+                     num_qubits, dtype, compiled=compiled)
 
 
 def sum_z_loss(theta: torch.Tensor, phi: torch.Tensor, num_qubits: int,
-               dtype: torch.dtype = torch.complex64) -> torch.Tensor:
+               dtype: torch.dtype = torch.complex64, compiled: bool = False) -> torch.Tensor:
     """Differentiable loss[b] = sum_i <Z_i> (float64) of the hardware-efficient
-    circuit on (batch, layers, n) angle tensors, with adjoint gradients."""
+    circuit on (batch, layers, n) angle tensors, with adjoint gradients
+    (compiled: see expectation)."""
     _check(theta, phi, num_qubits)
     n, L = num_qubits, theta.shape[1]
     return expectation(_ansatz_params(theta, phi), hardware_efficient_gates(n, L),
-                       sum_z_observable(n, theta.device), n, dtype)
+                       sum_z_observable(n, theta.device), n, dtype, compiled=compiled)

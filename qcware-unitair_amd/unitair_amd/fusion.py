@@ -554,6 +554,11 @@ def _pass_structs(items, n: int, tile_bits: int = TILE_BITS, reorder: bool = Tru
     return steps, op_structs, pending
 
 
+# structure key -> handle of its compiled pass (handles live as long as the
+# process; the library keeps the same table keyed by the generated source)
+_COMPILED = {}
+
+
 class FusedProgram:
     """A circuit compiled into fused passes for one (n, dtype, device).
 
@@ -656,10 +661,22 @@ class FusedProgram:
         if compiled:
             self._compile()
 
+    def _structure_key(self, i):
+        """What a compiled pass depends on: dtype, n, the pass record and the
+        kind and bits of each op (not the data pointers or strides)."""
+        fp, host = self._calls[i][1], self._calls[i][3]
+        count = fp.stage_ops[fp.num_stages]
+        ops = (FusedOp * max(1, count)).from_address(host)
+        return (self.code, self.n, bytes(fp),
+                tuple((ops[j].kind, ops[j].a, ops[j].b) for j in range(count)))
+
     def _compile(self):
         """Replace every pass with 12 tile bits by its compiled kernel
-        (ua_compile_fused_pass; the compiles run on parallel threads)."""
+        (ua_compile_fused_pass; new structures compile on parallel threads,
+        known ones come from a per-process table without a library call)."""
         todo = [i for i, c in enumerate(self._calls) if c[0] == "fused" and c[1].tile_bits == 12]
+        keys = {i: self._structure_key(i) for i in todo}
+        new = [i for i in todo if keys[i] not in _COMPILED]
 
         def one(i):
             fp, host = self._calls[i][1], self._calls[i][3]
@@ -668,12 +685,15 @@ class FusedProgram:
                                                 ctypes.c_void_p(host), ctypes.byref(handle))
             return rc, handle
 
-        workers = max(1, min(8, len(todo), os.cpu_count() or 1))
-        with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
-            results = list(ex.map(one, todo))
-        for i, (rc, handle) in zip(todo, results):
-            _lib.check(rc, "compiling a fused pass")
-            self._calls[i] = ("compiled", handle, self._calls[i][2])
+        if new:
+            workers = max(1, min(8, len(new), os.cpu_count() or 1))
+            with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+                results = list(ex.map(one, new))
+            for i, (rc, handle) in zip(new, results):
+                _lib.check(rc, "compiling a fused pass")
+                _COMPILED[keys[i]] = handle
+        for i in todo:
+            self._calls[i] = ("compiled", _COMPILED[keys[i]], self._calls[i][2])
         self.num_compiled = len(todo)
 
     def run(self, state: torch.Tensor) -> torch.Tensor:

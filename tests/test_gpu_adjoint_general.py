@@ -15,12 +15,13 @@ import adjoint_oracle as AO
 pytestmark = pytest.mark.gpu
 
 
-def _run(device, params, gates, obs, n, dtype, psi0=None):
+def _run(device, params, gates, obs, n, dtype, psi0=None, compiled=False):
     p = torch.from_numpy(params).to(device, torch.float64 if dtype == torch.complex128
                                     else torch.float32).requires_grad_(True)
     o = torch.from_numpy(obs).to(device).requires_grad_(True)
     init = None if psi0 is None else torch.from_numpy(psi0).to(device)
-    loss = adjoint.expectation(p, gates, o, n, dtype=dtype, initial_state=init)
+    loss = adjoint.expectation(p, gates, o, n, dtype=dtype, initial_state=init,
+                               compiled=compiled)
     loss.sum().backward()
     return loss.detach().cpu().numpy(), p.grad.double().cpu().numpy(), o.grad.cpu().numpy()
 
@@ -133,3 +134,27 @@ def test_general_adjoint_complex64_tolerance(device):
     scale = np.abs(obs).max()
     assert np.max(np.abs(l64 - l128)) < 2e-5 * scale
     assert np.max(np.abs(g64 - g128)) < 5e-5 * scale
+
+
+@pytest.mark.parametrize("n,dtype", [(13, torch.complex128), (14, torch.complex64)])
+def test_general_adjoint_compiled_passes(device, n, dtype):
+    """compiled=True (every pass a compiled kernel) against parameter shift
+    on the oracle (complex128) and against the interpreted passes; a second
+    call reuses the compiled kernels."""
+    rng = np.random.default_rng(n + 40)
+    P, B = 10, 2
+    gates = AO.random_rotation_circuit(rng, n, 60, P)
+    params = rng.uniform(-np.pi, np.pi, (B, P))
+    obs = rng.standard_normal(2 ** n)
+    loss, g, go = _run(device, params, gates, obs, n, dtype, compiled=True)
+    loss_i, g_i, go_i = _run(device, params, gates, obs, n, dtype)
+    scale = np.abs(obs).max()
+    tol = {torch.complex64: 2e-5, torch.complex128: 1e-11}[dtype] * scale
+    assert np.max(np.abs(loss - loss_i)) < tol and np.max(np.abs(g - g_i)) < tol
+    assert np.allclose(go, go_i, atol=tol)
+    if dtype == torch.complex128:
+        for i in range(P):
+            ps = AO.general_param_shift(params[0], gates, obs, n, i)
+            assert abs(g[0, i] - ps) < 1e-9, i
+    loss2, g2, _ = _run(device, params, gates, obs, n, dtype, compiled=True)
+    assert np.array_equal(loss2, loss) and np.array_equal(g2, g)
