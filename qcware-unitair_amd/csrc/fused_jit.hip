@@ -84,6 +84,40 @@ __device__ __forceinline__ T csel(bool s, T a, T b) { return cmk(s ? a.x : b.x, 
 __device__ __forceinline__ T expmi(R t) { R s, c; UA_SINCOS(t, &s, &c); return cmk(c, -s); }
 __device__ __forceinline__ T ldc(const SC R* p, int i) { return cmk(p[2 * i], p[2 * i + 1]); }
 
+// exp(-i t) for phase angles.  complex128: sincos.  complex64: Cody-Waite
+// reduction by pi/2 (pi/2 = P1 + P2 + P3; P1 has 8 significant bits, so k*P1
+// is exact for the |k| <= 163 used here, and each FMA rounds once), then the
+// Taylor polynomials of sin and cos on [-pi/4, pi/4] (truncation errors
+// below 2e-9) in Horner form.  Angles beyond 256 in magnitude (or NaN) take
+// sincosf.
+#if UA_DOUBLE
+__device__ __forceinline__ T expmi_phase(R t) { return expmi(t); }
+#else
+__device__ __forceinline__ T expmi_phase(float t) {
+  if (!(fabsf(t) <= 256.0f)) return expmi(t);
+  const float k = rintf(t * 0.636619772367581343f);
+  float r = fmaf(-k, 1.5703125f, t);
+  r = fmaf(-k, 4.83826792333275079e-4f, r);
+  r = fmaf(-k, 2.56334406825708965e-12f, r);
+  const float r2 = r * r;
+  float s = fmaf(r2, 2.755731922e-6f, -1.984126984e-4f);   // 1/9!, -1/7!
+  s = fmaf(r2, s, 8.333333333e-3f);                        // 1/5!
+  s = fmaf(r2, s, -1.666666667e-1f);                       // -1/3!
+  s = fmaf(r2 * r, s, r);
+  float c = fmaf(r2, -2.755731922e-7f, 2.480158730e-5f);   // -1/10!, 1/8!
+  c = fmaf(r2, c, -1.388888889e-3f);                       // -1/6!
+  c = fmaf(r2, c, 4.166666667e-2f);                        // 1/4!
+  c = fmaf(r2, c, -0.5f);
+  c = fmaf(r2, c, 1.0f);
+  // t = k pi/2 + r: sin t = (s, c, -s, -c)[k mod 4], cos t = (c, -s, -c, s)[k mod 4]
+  const int q = int(k) & 3;
+  float sn = (q & 1) ? c : s, cs = (q & 1) ? s : c;
+  if (q & 2) sn = -sn;
+  if ((q + 1) & 2) cs = -cs;
+  return cmk(cs, -sn);
+}
+#endif
+
 // Two consecutive elements (aligned to two elements), as native vectors.
 #if UA_DOUBLE
 typedef double v2r __attribute__((ext_vector_type(2)));
@@ -305,20 +339,26 @@ std::string generate(const PassShape& ps) {
     return e.str();
   };
   // diagonal ops on the pair (x0, x1) at state indices g, g | 1
-  auto diag_ops = [&](int st, const char* x0, const char* x1) {
+  // pk >= 0: the factors of PHASE_FACTORS ops were loaded ahead into
+  // pf<o>[2 pk], pf<o>[2 pk + 1] (emit_prefetch)
+  auto diag_ops = [&](int st, const char* x0, const char* x1, int pk = -1) {
     std::ostringstream e;
     for (int o = p.stage_ops[st]; o < p.stage_ops[st + 1]; ++o) {
       const ua_fused_op& op = ps.ops[o];
       switch (op.kind) {
         case UA_FOP_PHASE_FACTORS:
-          e << "      { T f0, f1; ld2((const GL T*)ops[" << o << "].data + b * ops[" << o
-            << "].bstride + g, f0, f1); " << x0 << " = cmul(f0, " << x0 << "); " << x1
-            << " = cmul(f1, " << x1 << "); }\n";
+          if (pk >= 0)
+            e << "      " << x0 << " = cmul(pf" << o << "[" << 2 * pk << "], " << x0 << "); "
+              << x1 << " = cmul(pf" << o << "[" << 2 * pk + 1 << "], " << x1 << ");\n";
+          else
+            e << "      { T f0, f1; ld2((const GL T*)ops[" << o << "].data + b * ops[" << o
+              << "].bstride + g, f0, f1); " << x0 << " = cmul(f0, " << x0 << "); " << x1
+              << " = cmul(f1, " << x1 << "); }\n";
           break;
         case UA_FOP_PHASE:
           e << "      { R a0, a1; ld2r((const GL R*)ops[" << o << "].data + b * ops[" << o
-            << "].bstride + g, a0, a1); " << x0 << " = cmul(expmi(a0), " << x0 << "); " << x1
-            << " = cmul(expmi(a1), " << x1 << "); }\n";
+            << "].bstride + g, a0, a1); " << x0 << " = cmul(expmi_phase(a0), " << x0 << "); "
+            << x1 << " = cmul(expmi_phase(a1), " << x1 << "); }\n";
           break;
         case UA_FOP_CZ:
           e << "      if ((g >> " << op.a << ") & (g >> " << op.b << ") & 1) " << x0
@@ -343,17 +383,31 @@ std::string generate(const PassShape& ps) {
     }
     return e.str();
   };
-  auto has_kind = [&](int st, int kind) {
-    for (int o = p.stage_ops[st]; o < p.stage_ops[st + 1]; ++o)
-      if (ps.ops[o].kind == kind) return true;
-    return false;
-  };
+
   // a diagonal stage on the registers of the global load / store
-  auto diag_regs = [&](int st) {
+  auto diag_regs = [&](int st, bool prefetched = false) {
     for (int k = 0; k < 8; ++k) {
       s << "  {\n    const u64 g = origin | " << hex64(depk[k]) << " | dep_t;\n";
       std::string a = "c[" + std::to_string(2 * k) + "]", c1 = "c[" + std::to_string(2 * k + 1) + "]";
-      s << diag_ops(st, a.c_str(), c1.c_str()) << "  }\n";
+      s << diag_ops(st, a.c_str(), c1.c_str(), prefetched ? k : -1) << "  }\n";
+    }
+  };
+  // Phase factors of a diagonal stage in tile (pair) order, loaded one stage
+  // ahead: barriers wait for LDS only, so the loads stay in flight while the
+  // stage before runs.
+  auto has_factors = [&](int st) {
+    for (int o = p.stage_ops[st]; o < p.stage_ops[st + 1]; ++o)
+      if (ps.ops[o].kind == UA_FOP_PHASE_FACTORS) return true;
+    return false;
+  };
+  auto emit_prefetch = [&](int st) {
+    for (int o = p.stage_ops[st]; o < p.stage_ops[st + 1]; ++o) {
+      if (ps.ops[o].kind != UA_FOP_PHASE_FACTORS) continue;
+      s << "  T pf" << o << "[16];\n";
+      for (int k = 0; k < 8; ++k)
+        s << "  ld2((const GL T*)ops[" << o << "].data + b * ops[" << o << "].bstride + (origin | "
+          << hex64(depk[k]) << " | dep_t), pf" << o << "[" << 2 * k << "], pf" << o << "["
+          << 2 * k + 1 << "]);\n";
     }
   };
 
@@ -363,120 +417,190 @@ std::string generate(const PassShape& ps) {
   const bool head = ns > 0 && p.stage_bits[0][0] < 0;
   const bool tail = ns > (head ? 1 : 0) && p.stage_bits[ns - 1][0] < 0;
   const int st_begin = head ? 1 : 0, st_end = ns - (tail ? 1 : 0);
+  // A cube stage that holds tile bit 0 (= state bit 0) takes its cube from
+  // global memory, and writes it back, as 8 pairs of adjacent amplitudes:
+  // lanes still read whole 128 B runs, and the stage needs no LDS round trip
+  // on that side.  Stages [lds_first, lds_end) go through LDS.
+  auto direct = [&](int st) { return p.stage_bits[st][0] == 0; };
+  const bool first_direct = st_begin < st_end && direct(st_begin);
+  const int lds_first = first_direct ? st_begin + 1 : st_begin;
+  const bool last_direct = st_end - 1 >= lds_first && direct(st_end - 1);
+  const int lds_end = last_direct ? st_end - 1 : st_end;
+  const bool uses_lds = st_begin < st_end && !(first_direct && lds_first >= st_end);
 
-  s << "  T c[16];\n";
-  for (int k = 0; k < 8; ++k)
-    s << "  ld2(st + (origin | " << hex64(depk[k]) << " | dep_t), c[" << 2 * k << "], c["
-      << 2 * k + 1 << "]);\n";
-  if (head) diag_regs(0);
-  if (st_begin < st_end) {
-    s << "  __shared__ T tile[4096];\n  char* const tb = reinterpret_cast<char*>(tile);\n";
-    for (int k = 0; k < 8; ++k)
-      s << "  tile[lds_t ^ " << ldsk[k] << "] = c[" << 2 * k << "]; tile[lds_t ^ " << ldsk[k]
-        << " ^ 1] = c[" << 2 * k + 1 << "];\n";
-    for (int st = st_begin; st < st_end; ++st) {
-      s << "  __syncthreads();\n";
-      if (p.stage_bits[st][0] < 0) {  // diagonal stage, LDS to LDS in tile order
-        s << "  {\n    constexpr u64 depk[8] = {";
-        for (int k = 0; k < 8; ++k) s << hex64(depk[k]) << (k < 7 ? ", " : "};\n");
-        s << (has_kind(st, UA_FOP_PHASE) ? "#pragma unroll 1\n" : "#pragma unroll\n")
-          << "    for (int k = 0; k < 8; ++k) {\n"
-          << "      const u64 g = origin | depk[k] | dep_t;\n"
-          << "      const int lp = lds_t ^ (512 * k) ^ (16 * (k & 1));\n"
-          << "      T x0 = tile[lp], x1 = tile[lp ^ 1];\n"
-          << diag_ops(st, "x0", "x1")
-          << "      tile[lp] = x0; tile[lp ^ 1] = x1;\n    }\n  }\n";
-        continue;
-      }
-      const int* sb = p.stage_bits[st];
-      int cb[4];  // state bits of the cube bits
-      for (int k = 0; k < 4; ++k) cb[k] = local[sb[k]];
-      auto slot_of = [&](int bit) {
-        for (int k = 0; k < 4; ++k)
-          if (cb[k] == bit) return k;
-        return -1;
-      };
-      // this thread's value of a state bit outside the cube
-      auto bit_of = [&](int bit) {
-        std::ostringstream e;
-        int tile_pos = -1;
-        for (int i = 0; i < kTileBits; ++i)
-          if (local[i] == bit) tile_pos = i;
-        if (tile_pos >= 0)
-          e << "(((e0 >> " << tile_pos << ") & 1) != 0)";
-        else
-          e << "(((origin >> " << bit << ") & 1) != 0)";
-        return e.str();
-      };
-      bool need_gb = false;
-      for (int o = p.stage_ops[st]; o < p.stage_ops[st + 1]; ++o)
-        need_gb = need_gb || ps.ops[o].kind == UA_FOP_CZ_LAYER;
-      s << "  {\n    using C = Cube<" << sb[0] << ", " << sb[1] << ", " << sb[2] << ", " << sb[3]
-        << ">;\n    const int e0 = C::e0(tid);\n"
-        << "    const int base = lds_pos(e0) * int(sizeof(T));\n"
-        << "    T q[16];\n    C::load(q, tb, base);\n";
-      if (need_gb) {
-        s << "    const u64 gb = origin";
-        for (int i = 0; i < kTileBits; ++i)
-          s << " | (u64((e0 >> " << i << ") & 1) << " << local[i] << ")";
-        s << ";\n";
-      }
-      for (int o = p.stage_ops[st]; o < p.stage_ops[st + 1]; ++o) {
-        const ua_fused_op& op = ps.ops[o];
-        switch (op.kind) {
-          case UA_FOP_DENSE1:
-            s << "    cube_1q<" << op.a << ">(q, " << gd(o) << ");\n";
-            break;
-          case UA_FOP_DENSE2:
-            s << "    cube_2q<" << op.a << ", " << op.b << ">(q, " << gd(o) << ");\n";
-            break;
-          case UA_FOP_CNOT: {
-            const int kc = slot_of(op.a);
-            if (kc >= 0)
-              s << "    cnot_cc<" << kc << ", " << op.b << ">(q);\n";
-            else
-              s << "    cnot_lane<" << op.b << ">(q, " << bit_of(op.a) << ");\n";
-            break;
-          }
-          case UA_FOP_CZ: {
-            const int ka = slot_of(op.a), kb = slot_of(op.b);
-            if (ka >= 0 && kb >= 0)
-              s << "    negate_cc<" << std::min(ka, kb) << ", " << std::max(ka, kb) << ">(q);\n";
-            else if (ka >= 0 || kb >= 0)
-              s << "    negate_c<" << (ka >= 0 ? ka : kb) << ">(q, "
-                << bit_of(ka >= 0 ? op.b : op.a) << ");\n";
-            else
-              s << "    negate_all(q, " << bit_of(op.a) << " && " << bit_of(op.b) << ");\n";
-            break;
-          }
-          case UA_FOP_DIAG1: {
-            const int k = slot_of(op.a);
-            if (k >= 0)
-              s << "    diag_c<" << k << ">(q, " << gd(o) << ");\n";
-            else
-              s << "    diag_all(q, " << gd(o) << ", " << bit_of(op.a) << ");\n";
-            break;
-          }
-          case UA_FOP_CZ_LAYER:
-            s << "    cz_layer<" << cb[0] << ", " << cb[1] << ", " << cb[2] << ", " << cb[3]
-              << ">(q, gb, ((const SC u64*)ops[" << o << "].data)[b * ops[" << o
-              << "].bstride]);\n";
-            break;
-          default:  // LAYER1Q heads: their DENSE1 sub-ops follow as ordinary ops
-            break;
-        }
-      }
-      s << "    C::store(q, tb, base);\n  }\n";
+  // one cube stage: its cube from LDS or global memory, its ops, and back to
+  // LDS or global memory (diagonal head / tail stages on the registers)
+  auto cube_stage = [&](int st, bool from_global, bool to_global) {
+    const int* sb = p.stage_bits[st];
+    int cb[4];  // state bits of the cube bits
+    for (int k = 0; k < 4; ++k) cb[k] = local[sb[k]];
+    auto slot_of = [&](int bit) {
+      for (int k = 0; k < 4; ++k)
+        if (cb[k] == bit) return k;
+      return -1;
+    };
+    // this thread's value of a state bit outside the cube
+    auto bit_of = [&](int bit) {
+      std::ostringstream e;
+      int tile_pos = -1;
+      for (int i = 0; i < kTileBits; ++i)
+        if (local[i] == bit) tile_pos = i;
+      if (tile_pos >= 0)
+        e << "(((e0 >> " << tile_pos << ") & 1) != 0)";
+      else
+        e << "(((origin >> " << bit << ") & 1) != 0)";
+      return e.str();
+    };
+    // state-index offset of cube entry j
+    auto dep_j = [&](int j) {
+      uint64_t d = 0;
+      for (int k = 0; k < 4; ++k)
+        if ((j >> k) & 1) d |= uint64_t(1) << cb[k];
+      return d;
+    };
+    bool need_gb = from_global || to_global;
+    for (int o = p.stage_ops[st]; o < p.stage_ops[st + 1]; ++o)
+      need_gb = need_gb || ps.ops[o].kind == UA_FOP_CZ_LAYER;
+    s << "  {\n    using C = Cube<" << sb[0] << ", " << sb[1] << ", " << sb[2] << ", " << sb[3]
+      << ">;\n    const int e0 = C::e0(tid);\n    T q[16];\n";
+    if (need_gb) {  // state index of cube entry 0
+      s << "    const u64 gb = origin";
+      for (int i = 0; i < kTileBits; ++i)
+        s << " | (u64((e0 >> " << i << ") & 1) << " << local[i] << ")";
+      s << ";\n";
     }
-    s << "  __syncthreads();\n";
+    if (!from_global || !to_global)
+      s << "    const int base = lds_pos(e0) * int(sizeof(T));\n";
+    if (from_global) {
+      for (int j = 0; j < 16; j += 2)
+        s << "    ld2(st + (gb | " << hex64(dep_j(j)) << "), q[" << j << "], q[" << j + 1
+          << "]);\n";
+      if (head)
+        for (int j = 0; j < 16; j += 2) {
+          s << "    {\n      const u64 g = gb | " << hex64(dep_j(j)) << ";\n";
+          std::string a = "q[" + std::to_string(j) + "]", c1 = "q[" + std::to_string(j + 1) + "]";
+          s << diag_ops(0, a.c_str(), c1.c_str()) << "    }\n";
+        }
+    } else {
+      s << "    C::load(q, tb, base);\n";
+    }
+    for (int o = p.stage_ops[st]; o < p.stage_ops[st + 1]; ++o) {
+      const ua_fused_op& op = ps.ops[o];
+      switch (op.kind) {
+        case UA_FOP_DENSE1:
+          s << "    cube_1q<" << op.a << ">(q, " << gd(o) << ");\n";
+          break;
+        case UA_FOP_DENSE2:
+          s << "    cube_2q<" << op.a << ", " << op.b << ">(q, " << gd(o) << ");\n";
+          break;
+        case UA_FOP_CNOT: {
+          const int kc = slot_of(op.a);
+          if (kc >= 0)
+            s << "    cnot_cc<" << kc << ", " << op.b << ">(q);\n";
+          else
+            s << "    cnot_lane<" << op.b << ">(q, " << bit_of(op.a) << ");\n";
+          break;
+        }
+        case UA_FOP_CZ: {
+          const int ka = slot_of(op.a), kb = slot_of(op.b);
+          if (ka >= 0 && kb >= 0)
+            s << "    negate_cc<" << std::min(ka, kb) << ", " << std::max(ka, kb) << ">(q);\n";
+          else if (ka >= 0 || kb >= 0)
+            s << "    negate_c<" << (ka >= 0 ? ka : kb) << ">(q, "
+              << bit_of(ka >= 0 ? op.b : op.a) << ");\n";
+          else
+            s << "    negate_all(q, " << bit_of(op.a) << " && " << bit_of(op.b) << ");\n";
+          break;
+        }
+        case UA_FOP_DIAG1: {
+          const int k = slot_of(op.a);
+          if (k >= 0)
+            s << "    diag_c<" << k << ">(q, " << gd(o) << ");\n";
+          else
+            s << "    diag_all(q, " << gd(o) << ", " << bit_of(op.a) << ");\n";
+          break;
+        }
+        case UA_FOP_CZ_LAYER:
+          s << "    cz_layer<" << cb[0] << ", " << cb[1] << ", " << cb[2] << ", " << cb[3]
+            << ">(q, gb, ((const SC u64*)ops[" << o << "].data)[b * ops[" << o
+            << "].bstride]);\n";
+          break;
+        default:  // LAYER1Q heads: their DENSE1 sub-ops follow as ordinary ops
+          break;
+      }
+    }
+    if (to_global) {
+      if (tail)
+        for (int j = 0; j < 16; j += 2) {
+          s << "    {\n      const u64 g = gb | " << hex64(dep_j(j)) << ";\n";
+          std::string a = "q[" + std::to_string(j) + "]", c1 = "q[" + std::to_string(j + 1) + "]";
+          s << diag_ops(ns - 1, a.c_str(), c1.c_str()) << "    }\n";
+        }
+      for (int j = 0; j < 16; j += 2)
+        s << "    st2(st + (gb | " << hex64(dep_j(j)) << "), q[" << j << "], q[" << j + 1
+          << "]);\n";
+    } else {
+      s << "    C::store(q, tb, base);\n";
+    }
+    s << "  }\n";
+  };
+
+  if (uses_lds)
+    s << "  __shared__ T tile[4096];\n  char* const tb = reinterpret_cast<char*>(tile);\n";
+  if (uses_lds && lds_first < lds_end && has_factors(lds_first)) emit_prefetch(lds_first);
+  else if (uses_lds && lds_first >= lds_end && !last_direct && tail && has_factors(ns - 1))
+    emit_prefetch(ns - 1);
+  if (first_direct) {
+    cube_stage(st_begin, true, !uses_lds);
+  } else {
+    s << "  T c[16];\n";
     for (int k = 0; k < 8; ++k)
-      s << "  c[" << 2 * k << "] = tile[lds_t ^ " << ldsk[k] << "]; c[" << 2 * k + 1
-        << "] = tile[lds_t ^ " << ldsk[k] << " ^ 1];\n";
+      s << "  ld2(st + (origin | " << hex64(depk[k]) << " | dep_t), c[" << 2 * k << "], c["
+        << 2 * k + 1 << "]);\n";
+    if (head) diag_regs(0);
+    if (uses_lds) {
+      for (int k = 0; k < 8; ++k)
+        s << "  tile[lds_t ^ " << ldsk[k] << "] = c[" << 2 * k << "]; tile[lds_t ^ " << ldsk[k]
+          << " ^ 1] = c[" << 2 * k + 1 << "];\n";
+    } else {  // diagonal stages only
+      if (tail) diag_regs(ns - 1);
+      for (int k = 0; k < 8; ++k)
+        s << "  st2(st + (origin | " << hex64(depk[k]) << " | dep_t), c[" << 2 * k << "], c["
+          << 2 * k + 1 << "]);\n";
+    }
   }
-  if (tail) diag_regs(ns - 1);
-  for (int k = 0; k < 8; ++k)
-    s << "  st2(st + (origin | " << hex64(depk[k]) << " | dep_t), c[" << 2 * k << "], c["
-      << 2 * k + 1 << "]);\n";
+  for (int st = lds_first; st < lds_end; ++st) {
+    s << "  __syncthreads();\n";
+    if (st + 1 < lds_end && has_factors(st + 1)) emit_prefetch(st + 1);
+    else if (st + 1 == lds_end && !last_direct && tail && has_factors(ns - 1))
+      emit_prefetch(ns - 1);
+    if (p.stage_bits[st][0] < 0) {  // diagonal stage, LDS to LDS in tile order
+      for (int k = 0; k < 8; ++k)
+        s << "  {\n    const u64 g = origin | " << hex64(depk[k]) << " | dep_t;\n"
+          << "    const int lp = lds_t ^ " << ldsk[k] << ";\n"
+          << "    T x0 = tile[lp], x1 = tile[lp ^ 1];\n"
+          << diag_ops(st, "x0", "x1", has_factors(st) ? k : -1)
+          << "    tile[lp] = x0; tile[lp ^ 1] = x1;\n  }\n";
+      continue;
+    }
+    cube_stage(st, false, false);
+  }
+  if (uses_lds) {
+    s << "  __syncthreads();\n";
+    if (last_direct) {
+      cube_stage(st_end - 1, false, true);
+    } else {
+      s << "  {\n    T c[16];\n";
+      for (int k = 0; k < 8; ++k)
+        s << "    c[" << 2 * k << "] = tile[lds_t ^ " << ldsk[k] << "]; c[" << 2 * k + 1
+          << "] = tile[lds_t ^ " << ldsk[k] << " ^ 1];\n";
+      if (tail) diag_regs(ns - 1, has_factors(ns - 1));
+      for (int k = 0; k < 8; ++k)
+        s << "    st2(st + (origin | " << hex64(depk[k]) << " | dep_t), c[" << 2 * k
+          << "], c[" << 2 * k + 1 << "]);\n";
+      s << "  }\n";
+    }
+  }
   s << "}\n";
   return s.str();
 }

@@ -87,17 +87,20 @@ def test_compiled_random_circuits(device, dt, n, batch):
     assert np.max(U.fidelity_error(ref, out)) < TOL[dt]
 
 
-@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
-def test_compiled_random_circuit_24q(device, dt):
+@pytest.mark.parametrize("dt,phase_angles", [(torch.complex64, False),
+                                             (torch.complex128, False),
+                                             (torch.complex64, True)])
+def test_compiled_random_circuit_24q(device, dt, phase_angles):
     from test_gpu_parity import _circuit_24q_and_oracle
     circ, psi, ref = _circuit_24q_and_oracle()
-    prog = FusedProgram(circ, dtype=dt, device=device, compiled=True)
+    prog = FusedProgram(circ, dtype=dt, device=device, compiled=True,
+                        phase_angles=phase_angles)
     assert prog.num_compiled == prog.num_passes
     out = prog.run(torch.from_numpy(psi).to(device, dt)).cpu().numpy()
     fid = float(U.fidelity_error(ref, out))
     mae = U.max_abs_error(ref, out)
-    print(f"compiled 24q {dt}: {prog.num_passes} passes, fidelity error {fid:.3e}, "
-          f"max-abs {mae:.3e}")
+    print(f"compiled 24q {dt} (phase angles: {phase_angles}): {prog.num_passes} passes, "
+          f"fidelity error {fid:.3e}, max-abs {mae:.3e}")
     bar = {torch.complex64: (1e-10, 1e-7), torch.complex128: (1e-14, 1e-14)}[dt]
     assert fid < bar[0] and mae < bar[1], (fid, mae)
 
@@ -160,3 +163,22 @@ def test_compiled_graph_replay_matches_eager(device):
         prog.replay()
         torch.cuda.synchronize()
         assert torch.equal(state, eager)
+
+
+def test_compiled_phase_angles_accuracy(device):
+    """A compiled complex64 pass that reads phase angles forms exp(-1j angle)
+    itself (reduction by pi/2 and polynomials up to |angle| 256, sincosf
+    beyond): against float64 exp on angles in [-300, 300] and special values."""
+    from unitair_amd.circuits import Circuit, Gate
+    n = 12
+    rng = np.random.default_rng(0)
+    ang = rng.uniform(-300, 300, 2 ** n).astype(np.float32)
+    ang[:10] = [0.0, np.pi / 2, -np.pi / 2, np.pi, 2 * np.pi, 256.0, -256.0, 256.5, 1e-30,
+                -3 * np.pi / 4]
+    ang[10:2000] = rng.uniform(0, 2 * np.pi, 1990).astype(np.float32)
+    circ = Circuit(n, [Gate("phase", (), angles=ang)])
+    prog = FusedProgram(circ, torch.complex64, device, compiled=True, phase_angles=True)
+    assert prog.num_compiled == prog.num_passes == 1
+    out = prog.run(torch.ones(1, 2 ** n, dtype=torch.complex64, device=device))
+    err = np.abs(out.cpu().numpy()[0] - np.exp(-1j * ang.astype(np.float64)))
+    assert np.max(err) < 3e-7, (int(np.argmax(err)), float(np.max(err)))
