@@ -105,6 +105,13 @@ def step_bytes_of(circ, amps, es=8, angle_es=4):
     return total
 
 
+def fused_step_bytes(circ, amps, passes, es=8):
+    """Algorithmic bytes of one fused step: a read and a write of the state per
+    pass, plus one read of each phase gate's precomputed factors (es bytes per
+    amplitude)."""
+    return passes * 2 * amps * es + circ.counts().get("phase", 0) * amps * es
+
+
 def step_latencies(runner, state, steps):
     """Latency of single circuit runs (synchronised on both sides):
     p50 / p99 / max in ms over `steps` runs."""
@@ -170,10 +177,13 @@ def per_gate_28q(dev, seed, n=28, steps=3, warmup=1):
     del runner
     prog = FusedProgram(circ, dtype=torch.complex64, device=dev)
     sec_f, _ = time_steps(prog, state, steps, warmup, with_events=False)
+    fb = fused_step_bytes(circ, amps, prog.num_passes)
     out["fused"] = {"passes": prog.num_passes, "ms_per_step": sec_f * 1e3,
                     "amp_updates_per_s": len(circ.gates) * amps / sec_f,
                     "ms_per_pass": sec_f * 1e3 / prog.num_passes,
-                    "pass_algorithmic_GBps": 2 * amps * 8 * prog.num_passes / sec_f / 1e9}
+                    "pass_algorithmic_GBps": 2 * amps * 8 * prog.num_passes / sec_f / 1e9,
+                    "step_algorithmic_bytes_with_phase_data": fb,
+                    "step_algorithmic_GBps": fb / sec_f / 1e9}
     del prog
     t0 = time.perf_counter()
     prog = FusedProgram(circ, dtype=torch.complex64, device=dev, compiled=True)
@@ -184,7 +194,11 @@ def per_gate_28q(dev, seed, n=28, steps=3, warmup=1):
                              "amp_updates_per_s": len(circ.gates) * amps / sec_c,
                              "ms_per_pass": sec_c * 1e3 / prog.num_passes,
                              "pass_algorithmic_GBps":
-                                 2 * amps * 8 * prog.num_passes / sec_c / 1e9}
+                                 2 * amps * 8 * prog.num_passes / sec_c / 1e9,
+                             "step_algorithmic_bytes_with_phase_data":
+                                 fused_step_bytes(circ, amps, prog.num_passes),
+                             "step_algorithmic_GBps":
+                                 fused_step_bytes(circ, amps, prog.num_passes) / sec_c / 1e9}
     del prog, state, circ
     torch.cuda.empty_cache()
     return out
@@ -745,6 +759,8 @@ def main():
         sec5 = time_graph(prog, state, args.steps, args.warmup)
         extra["fused"] = {"passes": prog.num_passes,
                           "ms_per_step": sec4 * 1e3,
+                          "step_algorithmic_GBps":
+                              fused_step_bytes(circ, amps, prog.num_passes) / sec4 / 1e9,
                           "amp_updates_per_s": ngates * amps / sec4,
                           "graph_ms_per_step": sec5 * 1e3,
                           "graph_amp_updates_per_s": ngates * amps / sec5,
@@ -763,6 +779,7 @@ def main():
             "passes": prog.num_passes, "compiled_passes": prog.num_compiled,
             "compile_s": compile_s, "ms_per_step": sec6 * 1e3,
             "amp_updates_per_s": ngates * amps / sec6,
+            "step_algorithmic_GBps": fused_step_bytes(circ, amps, prog.num_passes) / sec6 / 1e9,
             "graph_ms_per_step": sec7 * 1e3,
             "graph_amp_updates_per_s": ngates * amps / sec7,
             "ms_per_pass": sec6 * 1e3 / prog.num_passes,
