@@ -457,6 +457,10 @@ int ua_run_fused_pass(void* state, int dtype, int num_qubits, int64_t batch,
  */
 int ua_compile_fused_pass(int dtype, int num_qubits, const ua_fused_pass* pass,
                           const ua_fused_op* host_ops, void** handle);
+/* Loads a compiled pass's code on the current GPU (what the first
+ * ua_run_compiled_pass there does otherwise: call it before capturing that
+ * call into a HIP graph).  Allocates the code object's device memory. */
+int ua_load_compiled_pass(void* handle);
 /* Runs a compiled pass on `batch` contiguous states of the dtype and qubit
  * count it was compiled for; `ops` is a device op array of the compiled
  * structure. */

@@ -719,6 +719,42 @@ extern "C" int ua_compile_fused_pass(int dtype, int num_qubits, const ua_fused_p
   return UA_OK;
 }
 
+namespace ua {
+namespace {
+// The pass's function on the current device, loading its code there on first
+// use.
+int function_of(Compiled* c, hipFunction_t* f) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("hipGetDevice: %s", hipGetErrorString(e));
+    return UA_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(c->mu);
+  auto it = c->fn.find(dev);
+  if (it != c->fn.end()) {
+    *f = it->second;
+    return UA_OK;
+  }
+  hipModule_t mod;
+  e = hipModuleLoadData(&mod, c->code.data());
+  if (e == hipSuccess) e = hipModuleGetFunction(f, mod, "ua_compiled_pass");
+  if (e != hipSuccess) {
+    set_error("loading a compiled fused pass: %s", hipGetErrorString(e));
+    return UA_ERR_HIP;
+  }
+  c->fn[dev] = *f;
+  return UA_OK;
+}
+}  // namespace
+}  // namespace ua
+
+extern "C" int ua_load_compiled_pass(void* handle) {
+  UA_CHECK_ARG(handle != nullptr, "NULL handle");
+  hipFunction_t f = nullptr;
+  return ua::function_of(static_cast<ua::Compiled*>(handle), &f);
+}
+
 extern "C" int ua_run_compiled_pass(void* handle, void* state, int64_t batch,
                                     const ua_fused_op* ops, ua_stream_t stream) {
   UA_CHECK_ARG(handle != nullptr, "NULL handle");
@@ -730,32 +766,12 @@ extern "C" int ua_run_compiled_pass(void* handle, void* state, int64_t batch,
                "state must be aligned to two elements");
   const int64_t blocks = batch << (c->n - ua::kTileBits);
   UA_CHECK_ARG(batch <= (int64_t(0x7fffffff) >> (c->n - ua::kTileBits)), "too many tiles");
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) {
-    ua::set_error("hipGetDevice: %s", hipGetErrorString(e));
-    return UA_ERR_HIP;
-  }
   hipFunction_t f = nullptr;
-  {
-    std::lock_guard<std::mutex> lock(c->mu);
-    auto it = c->fn.find(dev);
-    if (it != c->fn.end()) {
-      f = it->second;
-    } else {
-      hipModule_t mod;
-      e = hipModuleLoadData(&mod, c->code.data());
-      if (e == hipSuccess) e = hipModuleGetFunction(&f, mod, "ua_compiled_pass");
-      if (e != hipSuccess) {
-        ua::set_error("loading a compiled fused pass: %s", hipGetErrorString(e));
-        return UA_ERR_HIP;
-      }
-      c->fn[dev] = f;
-    }
-  }
+  const int rc = ua::function_of(c, &f);
+  if (rc != UA_OK) return rc;
   void* args[] = {&state, &ops};
-  e = hipModuleLaunchKernel(f, unsigned(blocks), 1, 1, 256, 1, 1, 0,
-                            static_cast<hipStream_t>(stream), args, nullptr);
+  const hipError_t e = hipModuleLaunchKernel(f, unsigned(blocks), 1, 1, 256, 1, 1, 0,
+                                             static_cast<hipStream_t>(stream), args, nullptr);
   if (e != hipSuccess) {
     ua::set_error("launching a compiled fused pass: %s", hipGetErrorString(e));
     return UA_ERR_HIP;

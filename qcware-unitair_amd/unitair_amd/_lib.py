@@ -128,6 +128,7 @@ SIGNATURES["ua_run_circuit"] = (_i32, [_p, _p, _i32, _i32, _i64, ctypes.POINTER(
 SIGNATURES["ua_compile_fused_pass"] = (_i32, [_i32, _i32, ctypes.POINTER(FusedPass), _p,
                                               ctypes.POINTER(ctypes.c_void_p)])
 SIGNATURES["ua_run_compiled_pass"] = (_i32, [_p, _p, _i64, _p, _p])
+SIGNATURES["ua_load_compiled_pass"] = (_i32, [_p])
 SIGNATURES["ua_fused_pass_source"] = (_i32, [_i32, _i32, ctypes.POINTER(FusedPass), _p,
                                              ctypes.c_char_p, _i64, _pi64])
 UA_FUSE_MAX_OPS = 64

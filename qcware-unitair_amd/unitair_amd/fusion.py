@@ -692,8 +692,13 @@ class FusedProgram:
             for i, (rc, handle) in zip(new, results):
                 _lib.check(rc, "compiling a fused pass")
                 _COMPILED[keys[i]] = handle
-        for i in todo:
-            self._calls[i] = ("compiled", _COMPILED[keys[i]], self._calls[i][2])
+        with torch.cuda.device(self.device):
+            for i in todo:
+                # load the code on this GPU now, so that no launch (and no graph
+                # capture) has to
+                _lib.check(self.lib.ua_load_compiled_pass(_COMPILED[keys[i]]),
+                           "loading a compiled fused pass")
+                self._calls[i] = ("compiled", _COMPILED[keys[i]], self._calls[i][2])
         self.num_compiled = len(todo)
 
     def run(self, state: torch.Tensor) -> torch.Tensor:
