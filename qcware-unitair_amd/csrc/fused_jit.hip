@@ -418,9 +418,12 @@ std::string generate(const PassShape& ps) {
   const bool tail = ns > (head ? 1 : 0) && p.stage_bits[ns - 1][0] < 0;
   const int st_begin = head ? 1 : 0, st_end = ns - (tail ? 1 : 0);
   // A cube stage that holds tile bit 0 (= state bit 0) takes its cube from
-  // global memory, and writes it back, as 8 pairs of adjacent amplitudes:
-  // lanes still read whole 128 B runs, and the stage needs no LDS round trip
-  // on that side.  Stages [lds_first, lds_end) go through LDS.
+  // global memory, and writes it back, as 8 pairs of adjacent amplitudes, and
+  // needs no LDS round trip on that side.  The lanes' index bits fill the
+  // lowest tile bits outside the cube, so a 128 B line is read whole by one
+  // load instruction of 8 lanes (tile bits 1..3 outside the cube) or by a
+  // few consecutive loads of the same lanes.  Stages [lds_first, lds_end) go
+  // through LDS.
   auto direct = [&](int st) { return p.stage_bits[st][0] == 0; };
   const bool first_direct = st_begin < st_end && direct(st_begin);
   const int lds_first = first_direct ? st_begin + 1 : st_begin;

@@ -136,8 +136,10 @@ def _z0_diag(n: int, device) -> torch.Tensor:
 class _Lane:
     NBUF = 2   # steps in flight per lane (gate staging buffers)
 
-    def __init__(self, n: int, capacity: int, kmax: int, device, use_graph: bool):
+    def __init__(self, n: int, capacity: int, kmax: int, device, use_graph: bool,
+                 compiled: bool = False):
         self.n = n
+        self.compiled = compiled
         self.capacity = capacity
         self.kmax = kmax
         self.device = device
@@ -194,7 +196,7 @@ class _Lane:
             # order whatever K and the other slots, so its result is the same
             # alone and in the mix
             prog = FusedProgram.from_items(items, self.n, torch.complex64, self.device,
-                                           reorder=False)
+                                           reorder=False, compiled=self.compiled)
             # per-slot data: consecutive slots are 4 (2x2) elements apart in
             # mats[buf][l, q] and 1 mask apart in masks[buf][l], the defaults
             view = self.states[:used]
@@ -274,13 +276,18 @@ class Server:
     max_active limits the requests on the GPU at once (None: slots and the
     HBM budget decide); max_active_amplitudes limits the sum of 2^n over the
     active requests (None: no limit); max_slots=1, max_active=1 runs one
-    request at a time on the same fused programs."""
+    request at a time on the same fused programs.  compiled: lanes of 12 or
+    more qubits run compiled passes (FusedProgram(compiled=True)); the first
+    step of each (qubit count, layers per step) compiles its passes on the
+    host, about a second each, so this suits a steady mix of circuit shapes
+    (compiled passes are kept for the life of the process)."""
 
     def __init__(self, hbm_budget: int = 64 << 30, max_slots: int = 32,
                  max_layers_per_step: int = 8, device="cuda", use_graphs: bool = True,
                  max_active: Optional[int] = None,
-                 max_active_amplitudes: Optional[int] = 1 << 22):
+                 max_active_amplitudes: Optional[int] = 1 << 22, compiled: bool = False):
         self.budget = hbm_budget
+        self.compiled = compiled
         self.max_slots = max_slots
         self.kmax = max_layers_per_step
         self.device = torch.device(device)
@@ -316,7 +323,8 @@ class Server:
                     return False
                 need = cap * r.footprint
             with on_device(self.device):
-                lane = _Lane(r.n, int(cap), self.kmax, self.device, self.use_graphs)
+                lane = _Lane(r.n, int(cap), self.kmax, self.device, self.use_graphs,
+                             self.compiled)
             self.lanes[r.n] = lane
             self.used_bytes += need
         slot = lane.free_slot()

@@ -98,3 +98,15 @@ def test_admission_while_lanes_run_and_fused_baseline(device):
         assert a.done >= a.arrival and b.done >= b.arrival
         assert abs(a.norm2 - b.norm2) < 1e-6 and abs(a.z0 - b.z0) < 1e-6
     assert not server.lanes and server.used_bytes == 0 and server.active == 0
+
+
+def test_compiled_lanes_match_oracle(device):
+    """Server(compiled=True): 13- and 14-qubit lanes on compiled passes
+    (captured into graphs) against the complex128 oracle."""
+    reqs = [Request(i, n, d, seed=500 + i) for i, (n, d) in
+            enumerate([(13, 8), (14, 8), (13, 8)])]
+    Server(device=device, max_layers_per_step=4, compiled=True).run(reqs, realtime=False)
+    for r in reqs:
+        norm, z0 = oracle_answer(r)
+        assert abs(r.norm2 - 1.0) < 1e-5
+        assert abs(r.z0 - z0) < 1e-5, (r.n, r.depth)
