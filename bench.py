@@ -277,7 +277,8 @@ def adjoint_bench(dev, iters=3):
         adjoint.sum_z_loss(t, p, n, dtype=torch.complex64).sum().backward()
         return t.grad, p.grad
 
-    adj_step(th, ph)
+    for _ in range(2):          # the first calls also plan, allocate and ramp clocks
+        adj_step(th, ph)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(iters):
@@ -297,6 +298,7 @@ def adjoint_bench(dev, iters=3):
     g_c = adj_step_compiled(th, ph)         # the first call compiles the passes
     torch.cuda.synchronize()
     first_compiled_s = time.perf_counter() - t0
+    adj_step_compiled(th, ph)
     compiled_vs_interp = max(float((a - b).abs().max()) for a, b in zip(g_ref, g_c))
     t0 = time.perf_counter()
     for _ in range(iters):

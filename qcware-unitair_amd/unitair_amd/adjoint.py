@@ -32,7 +32,7 @@ import torch
 
 from . import _lib
 from ._device import on_device
-from .fusion import FusedProgram, _Item
+from .fusion import FusedProgram, PlanTemplate, _Item
 
 _REAL = {torch.complex64: torch.float32, torch.complex128: torch.float64}
 _PAULI = {"x": _lib.UA_PAULI_X, "y": _lib.UA_PAULI_Y, "z": _lib.UA_PAULI_Z}
@@ -57,6 +57,7 @@ class _Spec:
     def __init__(self, gates: Sequence[tuple], n: int, num_params: int):
         self.n = n
         self.compiled = False
+        self.key = None
         segments, fixed, rot = [], [], []
         for g in gates:
             kind = g[0]
@@ -141,8 +142,44 @@ def _rot_items(s, rot, table, n):
             for r, (kind, q, idx, coef) in enumerate(rot)]
 
 
-def _program(items, n, dtype, device, compiled=False):
-    return FusedProgram.from_items(items, n, dtype, device, compiled=compiled) if items else None
+# Plans of the adjoint's programs, by gate-list structure and program: each
+# call builds fresh programs (its own op records and data) from these
+# without planning again.  Oldest entries are dropped past _TEMPLATE_LIMIT.
+_TEMPLATES = {}
+_TEMPLATE_LIMIT = 256
+
+
+_KEY_IDS = {}
+
+
+def _gates_key(gates, n: int, num_params: int) -> int:
+    """A small id for everything of a gate list that the plans depend on
+    (array operands by shape only)."""
+    key = []
+    for g in gates:
+        parts = [g[0]]
+        for x in g[1:]:
+            if isinstance(x, (bool, int, float, str, np.integer, np.floating)):
+                parts.append(x.item() if isinstance(x, (np.integer, np.floating)) else x)
+            else:
+                parts.append(("array", tuple(np.shape(x))))
+        key.append(tuple(parts))
+    return _KEY_IDS.setdefault((n, num_params, tuple(key)), len(_KEY_IDS))
+
+
+def _program(items, n, dtype, device, compiled=False, key=None):
+    if not items:
+        return None
+    template = None
+    if key is not None:
+        template = _TEMPLATES.get(key)
+        if template is None:
+            template = PlanTemplate(items, n)
+            if len(_TEMPLATES) >= _TEMPLATE_LIMIT:
+                _TEMPLATES.pop(next(iter(_TEMPLATES)))
+            _TEMPLATES[key] = template
+    return FusedProgram.from_items(items, n, dtype, device, compiled=compiled,
+                                   template=template)
 
 
 def _check_inputs(params, observable, n, dtype, device):
@@ -176,18 +213,21 @@ class _Programs:
             for s in range(c0, min(S, c0 + self.every)):
                 fixed, rot = spec.segments[s]
                 items += list(fixed) + _rot_items(s, rot, fwd_table, n)
-            self.fwd.append((c0, _program(items, n, dtype, dev, spec.compiled)))
-        self.trail = _program(list(spec.trailing), n, dtype, dev, spec.compiled)
+            self.fwd.append((c0, _program(items, n, dtype, dev, spec.compiled,
+                                          (spec.key, "fwd", c0))))
+        self.trail = _program(list(spec.trailing), n, dtype, dev, spec.compiled,
+                              (spec.key, "trail"))
         self.inv_trail = self.inv = None
         if backward:
             inv_table = _rotation_table(spec, p, dtype, -1.0, repeat=2)
             self.inv_trail = _program([_inverse_fixed(it) for it in reversed(spec.trailing)],
-                                      n, dtype, dev, spec.compiled)
+                                      n, dtype, dev, spec.compiled, (spec.key, "inv_trail"))
             self.inv = []
             for s, (fixed, rot) in enumerate(spec.segments):
                 items = _rot_items(s, rot, inv_table, n)
                 items += [_inverse_fixed(it) for it in reversed(fixed)]
-                self.inv.append(_program(items, n, dtype, dev, spec.compiled))
+                self.inv.append(_program(items, n, dtype, dev, spec.compiled,
+                                         (spec.key, "inv", s)))
 
 
 # Algorithmic HBM bytes of the last forward and backward (state passes, copies,
@@ -321,6 +361,7 @@ def expectation(params: torch.Tensor, gates: Sequence[tuple], observable: torch.
     _check_inputs(params, observable, n, dtype, params.device)
     spec = _Spec(gates, n, params.shape[1])
     spec.compiled = compiled
+    spec.key = _gates_key(gates, n, params.shape[1])
     return _AdjointExpectation.apply(params, observable, spec, dtype, initial_state)
 
 
@@ -341,7 +382,8 @@ def run_gates(params: torch.Tensor, gates: Sequence[tuple], num_qubits: int,
         for s, (fixed, rot) in enumerate(spec.segments):
             items += list(fixed) + _rot_items(s, rot, table, n)
         items += list(spec.trailing)
-        prog = _program(items, n, dtype, dev, compiled)
+        prog = _program(items, n, dtype, dev, compiled,
+                        (_gates_key(gates, n, params.shape[1]), "run"))
         psi = _initial(B, n, dtype, dev, initial_state)
         if prog is not None:
             prog.run(psi)

@@ -48,6 +48,9 @@ class _Item:
     bits: tuple                      # state bits (dense: gate MSB first)
     data: Optional[np.ndarray] = None
     per_state: bool = False          # data has a leading batch dim
+    # how the planner derived this item from its input items (None: it is
+    # one of them): ("as_dense1", item) or ("product", later, earlier)
+    src: Optional[tuple] = field(default=None, repr=False, compare=False)
 
 
 @dataclass
@@ -118,8 +121,12 @@ def _segment_stage(ops: List[_Item], cube_state_bits) -> list:
                 run = [[] for _ in range(len(cube_state_bits))]
             lane = run[cube_pos[it.bits[0]]]
             g = _as_dense1(it)
+            if g is not it:
+                g.src = ("as_dense1", it)
             if lane:
-                lane[-1] = _product(g, lane[-1])
+                prod = _product(g, lane[-1])
+                prod.src = ("product", g, lane[-1])
+                lane[-1] = prod
             else:
                 lane.append(g)
             continue
@@ -554,6 +561,48 @@ def _pass_structs(items, n: int, tile_bits: int = TILE_BITS, reorder: bool = Tru
     return steps, op_structs, pending
 
 
+class PlanTemplate:
+    """The plan of a gate-item list with its data left open: the kernel
+    structures of _pass_structs, and for each op that reads data a recipe in
+    terms of the input items' positions ("item", i) | ("as_dense1", r) |
+    ("product", r_later, r_earlier).  FusedProgram.from_items(items,
+    template=t) builds a program for any item list of the same kinds, bits
+    and per-state flags without planning again."""
+
+    def __init__(self, items, n: int, tile_bits: int = TILE_BITS, reorder: bool = True):
+        items = list(items)
+        self.n, self.tile_bits, self.reorder = n, tile_bits, reorder
+        self.signature = [(it.kind, tuple(it.bits), bool(it.per_state)) for it in items]
+        index = {id(it): i for i, it in enumerate(items)}
+
+        def recipe(it):
+            if it.src is None:
+                return ("item", index[id(it)])
+            if it.src[0] == "as_dense1":
+                return ("as_dense1", recipe(it.src[1]))
+            return ("product", recipe(it.src[1]), recipe(it.src[2]))
+
+        steps, self.op_structs, pending = _pass_structs(items, n, tile_bits, reorder)
+        self.steps = [st if st[0] == "fused" else ("wide", st[1], index[id(st[2])])
+                      for st in steps]
+        self.pending = [(idx, recipe(it)) for idx, it in pending]
+
+    def check(self, items):
+        sig = [(it.kind, tuple(it.bits), bool(it.per_state)) for it in items]
+        if sig != self.signature:
+            raise ValueError("the items differ from the template's in kind, bits or "
+                             "per-state data")
+
+    @staticmethod
+    def evaluate(recipe, items):
+        if recipe[0] == "item":
+            return items[recipe[1]]
+        if recipe[0] == "as_dense1":
+            return _as_dense1(PlanTemplate.evaluate(recipe[1], items))
+        return _product(PlanTemplate.evaluate(recipe[1], items),
+                        PlanTemplate.evaluate(recipe[2], items))
+
+
 # structure key -> handle of its compiled pass (handles live as long as the
 # process; the library keeps the same table keyed by the generated source)
 _COMPILED = {}
@@ -579,14 +628,19 @@ class FusedProgram:
     def from_items(cls, items: List[_Item], n: int, dtype: torch.dtype = torch.complex64,
                    device="cuda", tile_bits: int = TILE_BITS,
                    reorder: bool = True, compiled: bool = False,
-                   phase_angles: bool = False) -> "FusedProgram":
+                   phase_angles: bool = False,
+                   template: Optional[PlanTemplate] = None) -> "FusedProgram":
+        """template: a PlanTemplate of an item list with the same kinds, bits
+        and per-state flags (its n, tile_bits and reorder apply); the plan is
+        then taken from it instead of planned again."""
         prog = cls.__new__(cls)
-        prog._build(list(items), n, dtype, device, tile_bits, reorder, compiled, phase_angles)
+        prog._build(list(items), n, dtype, device, tile_bits, reorder, compiled, phase_angles,
+                    template)
         prog.num_gates = len(items)
         return prog
 
     def _build(self, items, n, dtype, device, tile_bits, reorder=True, compiled=False,
-               phase_angles=False):
+               phase_angles=False, template=None):
         if dtype not in (torch.complex64, torch.complex128):
             raise ValueError("fused programs run complex64/complex128 states")
         for it in items:
@@ -602,12 +656,20 @@ class FusedProgram:
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         real = torch.float32 if dtype == torch.complex64 else torch.float64
-        steps, op_structs, pending = _pass_structs(items, self.n, tile_bits, reorder)
+        if template is None:
+            template = PlanTemplate(items, self.n, tile_bits, reorder)
+        else:
+            if template.n != self.n:
+                raise ValueError("the template was planned for another qubit count")
+            template.check(items)
+        # fresh op records per program (data pointers differ between programs)
+        op_structs = [FusedOp.from_buffer_copy(o) for o in template.op_structs]
+        pending = [(idx, PlanTemplate.evaluate(r, items)) for idx, r in template.pending]
         self._keep = []
         self.steps = []
-        for st in steps:
+        for st in template.steps:
             if st[0] == "wide":
-                m = torch.from_numpy(st[2].data).to(self.device, dtype)
+                m = torch.from_numpy(np.asarray(items[st[2]].data)).to(self.device, dtype)
                 self._keep.append(m)
                 self.steps.append(("wide", st[1], m))
             else:

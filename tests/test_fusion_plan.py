@@ -238,3 +238,45 @@ def test_in_order_plan_keeps_the_gate_order(n, seed):
         assert seen == p.items
         flat += p.items
     assert [id(x) for x in flat] == [id(x) for x in items]
+
+
+def _template_matches_plan(items, n):
+    from unitair_amd.fusion import PlanTemplate, _pass_structs
+    t = PlanTemplate(items, n)
+    steps, ops, pending = _pass_structs(items, n)
+    assert [bytes(o) for o in ops] == [bytes(o) for o in t.op_structs]
+    assert [i for i, _ in pending] == [i for i, _ in t.pending]
+    for (_, it), (_, r) in zip(pending, t.pending):
+        v = PlanTemplate.evaluate(r, items)
+        assert v.kind == it.kind and v.bits == it.bits
+        if it.data is None:
+            assert v.data is None
+        else:
+            assert np.array_equal(np.asarray(v.data), np.asarray(it.data))
+    return t
+
+
+@pytest.mark.parametrize("n,seed", [(8, 1), (12, 2), (14, 3), (20, 4)])
+def test_plan_template_reproduces_the_plan(n, seed):
+    """A PlanTemplate's recipes (input items, their diagonals as 2x2
+    matrices, products of runs on one bit) give back exactly the data the
+    planner attaches to each op, for mixed circuits and for runs of
+    diagonal and dense 1-qubit gates on shared qubits."""
+    from unitair_amd.fusion import _Item, _items_from_circuit, _merge_1q
+    items = _merge_1q(_items_from_circuit(random_mixed_circuit(n, 80, seed)))
+    _template_matches_plan(items, n)
+    rng = np.random.default_rng(seed)
+    items = []
+    for layer in range(5):
+        for q in range(n):
+            items.append(_Item("diag1", (q,), np.exp(1j * rng.uniform(0, 6, 2))))
+            items.append(_Item("dense1", (q,), haar_unitary(2, rng)))
+        items.append(_Item("cz", (layer % n, (layer + 1) % n)))
+    t = _template_matches_plan(items, n)
+    # the same kinds and bits with other data: accepted; other bits: refused
+    other = [_Item(it.kind, it.bits, None if it.data is None else np.asarray(it.data) * 1j)
+             for it in items]
+    t.check(other)
+    other[0] = _Item("diag1", ((other[0].bits[0] + 1) % n,), other[0].data)
+    with pytest.raises(ValueError, match="differ from the template"):
+        t.check(other)
