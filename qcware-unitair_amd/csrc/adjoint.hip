@@ -101,6 +101,104 @@ __global__ __launch_bounds__(kBlock) void grad_low_kernel(
   }
 }
 
+// pair_term in the state's precision, returned as double (the sums are
+// float64): complex64 terms are formed in float32 (relative rounding 6e-8
+// per term, far inside complex64 state error), complex128 in double.
+template <int P>
+__device__ __forceinline__ double pair_term_of(float2 l0, float2 l1, float2 p0, float2 p1) {
+  float r;
+  if constexpr (P == UA_PAULI_X)
+    r = (l0.x * p1.y - l0.y * p1.x) + (l1.x * p0.y - l1.y * p0.x);
+  else
+    r = (l1.x * p0.x + l1.y * p0.y) - (l0.x * p1.x + l0.y * p1.y);
+  return double(r);
+}
+template <int P>
+__device__ __forceinline__ double pair_term_of(double2 l0, double2 l1, double2 p0, double2 p1) {
+  return pair_term<P>(l0, l1, p0, p1);
+}
+
+// The same sums for a full 4096-amplitude tile (m = 12, every state of 12 or
+// more qubits) with the work in registers: after a coalesced copy of both
+// tiles into LDS, each thread reads three 16-amplitude cubes (tile bits 0..3,
+// 4..7, 8..11) and adds the pair terms of the cube's 4 bits; the 12 sums stay
+// in registers until one reduction at the end.  LDS position of element e:
+// e ^ ((e >> 4) & 15), so the copy and all three cube reads touch distinct
+// banks across 16 lanes.
+__device__ __forceinline__ int low_swz(int e) { return e ^ ((e >> 4) & 15); }
+
+template <int P, int B0, typename T>
+__device__ __forceinline__ void low_cube(double (&acc)[kLowBits], const T* sl, const T* sp,
+                                         int e0) {
+  T l[16], p[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    l[j] = sl[low_swz(e0 | (j << B0))];
+    p[j] = sp[low_swz(e0 | (j << B0))];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (j & (1 << k)) continue;
+      const int j1 = j | (1 << k);
+      acc[B0 + k] += pair_term_of<P>(l[j], l[j1], p[j], p[j1]);
+    }
+}
+
+// complex64 tiles (64 KiB of LDS) leave room for two workgroups per CU:
+// the register budget asks for both
+template <typename T>
+constexpr int low_full_waves() { return sizeof(T) == 8 ? 2 : 1; }
+
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(low_full_waves<T>()))) void grad_low_full_kernel(
+    double* __restrict__ partial, const T* __restrict__ lam, const T* __restrict__ psi, int n,
+    int64_t tiles_per_state) {
+  __shared__ T sl[kLowTile];
+  __shared__ T sp[kLowTile];
+  __shared__ double red[kBlock / 64][kLowBits];
+  const int64_t tile = blockIdx.x;
+  const int64_t b = tile / tiles_per_state;
+  const int64_t t = tile % tiles_per_state;
+  const int64_t N = int64_t(1) << n;
+  const int tid = threadIdx.x;
+  const T* L = lam + b * N + t * kLowTile;
+  const T* Ps = psi + b * N + t * kLowTile;
+#pragma unroll
+  for (int k = 0; k < kLowTile / kBlock; ++k) {
+    const int i = tid + k * kBlock;
+    sl[low_swz(i)] = L[i];
+    sp[low_swz(i)] = Ps[i];
+  }
+  __syncthreads();
+  double acc[kLowBits];
+#pragma unroll
+  for (int i = 0; i < kLowBits; ++i) acc[i] = 0.0;
+  // thread tid's cube entry 0: tid's 8 bits around the cube's 4
+  // (the empty asm keeps each cube's LDS reads after the previous cube's
+  // arithmetic: with all three cubes' reads hoisted the registers overflow)
+  low_cube<P, 0>(acc, sl, sp, tid << 4);
+  asm volatile("" ::: "memory");
+  low_cube<P, 4>(acc, sl, sp, (tid & 15) | ((tid >> 4) << 8));
+  asm volatile("" ::: "memory");
+  low_cube<P, 8>(acc, sl, sp, tid);
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int i = 0; i < kLowBits; ++i) {
+    const double v = wave_sum(acc[i]);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  if (tid < kLowBits) {
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) r += red[w][tid];
+    partial[(b * tiles_per_state + t) * n + tid] = r;
+  }
+}
+
 // X / Y gradients for a window of H index bits w0..w0+H-1 (w0 >= 12).  Each
 // thread owns one low index (bits 0..11: 256 consecutive per workgroup, so
 // every load is coalesced) and, for each value r of the other high bits in
@@ -204,17 +302,19 @@ __global__ __launch_bounds__(kBlock) void grad_z_kernel(
   }
 }
 
-// out[b * n_out + q] = sum over `parts` rows of partial[(b*parts + r)*stride + q0 + q]
+// out[b * out_stride + out_q0 + out_dir * q] = sum over `parts` rows of
+// partial[(b*parts + r)*stride + q0 + q], q < n_out (out_dir -1: index bit
+// q0 + q is qubit out_q0 - q)
 __global__ __launch_bounds__(kBlock) void sum_parts_kernel(
     double* __restrict__ out, const double* __restrict__ partial, int64_t batch, int64_t parts,
-    int stride, int q0, int n_out, int out_stride, int out_q0, int accumulate) {
+    int stride, int q0, int n_out, int out_stride, int out_q0, int accumulate, int out_dir) {
   const int64_t idx = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= batch * n_out) return;
   const int64_t b = idx / n_out;
   const int q = int(idx % n_out);
   double s = 0.0;
   for (int64_t r = 0; r < parts; ++r) s += partial[(b * parts + r) * stride + q0 + q];
-  double* o = out + b * out_stride + out_q0 + q;
+  double* o = out + b * out_stride + out_q0 + out_dir * q;
   *o = accumulate ? *o + s : s;
 }
 
@@ -298,7 +398,7 @@ extern "C" int ua_pauli_gradients(double* out, const void* lambda, const void* p
     if (rc) return rc;
     // out[b*n + q] = sum over chunks (partials are stored by qubit)
     sum_parts_kernel<<<grid_for(batch * n), kBlock, 0, s>>>(out, part, batch, chunks, n + 1, 0, n,
-                                                            n, 0, 0);
+                                                            n, 0, 0, 1);
     return check_launch("sum_parts_kernel");
   }
   // X / Y: index bit i belongs to qubit n-1-i; partials are kept per index
@@ -307,9 +407,15 @@ extern "C" int ua_pauli_gradients(double* out, const void* lambda, const void* p
   const int64_t tiles = y_low_parts(n);
   UA_CHECK_ARG(batch * tiles <= 0x7fffffffLL, "grid too large");
   const unsigned glow = unsigned(batch * tiles);
-#define UA_LOW(TT, PP)                                                                  \
-  grad_low_kernel<TT, PP><<<glow, kBlock, 0, s>>>(part, (const TT*)lambda, (const TT*)psi, n, \
-                                                  m, tiles)
+#define UA_LOW(TT, PP)                                                                   \
+  do {                                                                                   \
+    if (m == kLowBits)                                                                   \
+      grad_low_full_kernel<TT, PP><<<glow, kBlock, 0, s>>>(part, (const TT*)lambda,      \
+                                                           (const TT*)psi, n, tiles);     \
+    else                                                                                 \
+      grad_low_kernel<TT, PP><<<glow, kBlock, 0, s>>>(part, (const TT*)lambda,           \
+                                                      (const TT*)psi, n, m, tiles);       \
+  } while (0)
   if (pauli == UA_PAULI_X) {
     if (c64) UA_LOW(float2, UA_PAULI_X); else UA_LOW(double2, UA_PAULI_X);
   } else {
@@ -318,13 +424,11 @@ extern "C" int ua_pauli_gradients(double* out, const void* lambda, const void* p
 #undef UA_LOW
   int rc = check_launch("grad_low_kernel");
   if (rc) return rc;
-  // bits 0..m-1 -> qubits n-1 .. n-m   (sum_parts writes q-th bit at out index)
-  for (int bit = 0; bit < m; ++bit) {
-    sum_parts_kernel<<<grid_for(batch), kBlock, 0, s>>>(out, part, batch, tiles, n, bit, 1, n,
-                                                        n - 1 - bit, 0);
-    rc = check_launch("sum_parts_kernel");
-    if (rc) return rc;
-  }
+  // bits 0..m-1 -> qubits n-1 .. n-m, one launch
+  sum_parts_kernel<<<grid_for(batch * m), kBlock, 0, s>>>(out, part, batch, tiles, n, 0, m, n,
+                                                          n - 1, 0, -1);
+  rc = check_launch("sum_parts_kernel");
+  if (rc) return rc;
   // high bits, kHighWindow at a time; each launch reads both states once
   UA_CHECK_ARG(batch <= 0x7fffffffLL, "grid too large");
   const int64_t z = high_z(n, batch);
@@ -356,12 +460,11 @@ extern "C" int ua_pauli_gradients(double* out, const void* lambda, const void* p
 #undef UA_HIGH
     rc = check_launch("grad_high_kernel");
     if (rc) return rc;
-    for (int bit = w0; bit < w0 + H; ++bit) {
-      sum_parts_kernel<<<grid_for(batch), kBlock, 0, s>>>(out, part, batch, kHighParts * zz, n,
-                                                          bit, 1, n, n - 1 - bit, 0);
-      rc = check_launch("sum_parts_kernel");
-      if (rc) return rc;
-    }
+    // bits w0..w0+H-1 -> qubits n-1-w0 .. n-w0-H, one launch
+    sum_parts_kernel<<<grid_for(batch * H), kBlock, 0, s>>>(out, part, batch, kHighParts * zz, n,
+                                                            w0, H, n, n - 1 - w0, 0, -1);
+    rc = check_launch("sum_parts_kernel");
+    if (rc) return rc;
   }
   return UA_OK;
 }
@@ -452,7 +555,7 @@ extern "C" int ua_expect_sum_z(double* out, const void* psi, int dtype, int num_
   int rc = check_launch("grad_z_kernel");
   if (rc) return rc;
   sum_parts_kernel<<<grid_for(batch), kBlock, 0, s>>>(out, part, batch, chunks, n + 1, n, 1, 1, 0,
-                                                      0);
+                                                      0, 1);
   return check_launch("sum_parts_kernel");
 }
 
