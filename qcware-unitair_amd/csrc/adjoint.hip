@@ -257,7 +257,6 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void grad_z_kernel(
     double* __restrict__ partial, const T* __restrict__ lam, const T* __restrict__ psi, int n,
     int64_t chunks, int want_loss) {
-  __shared__ double scratch[kBlock / 64];
   constexpr int K = kZChunkConst / kBlock;  // 16
   const int64_t b = blockIdx.x / chunks;
   const int64_t c = blockIdx.x % chunks;
@@ -281,24 +280,45 @@ __global__ __launch_bounds__(kBlock) void grad_z_kernel(
   double total = 0.0;
 #pragma unroll
   for (int k = 0; k < K; ++k) total += vk[k];
-  if (want_loss) {
-    const double sl = block_sum(loss, scratch);
-    if (threadIdx.x == 0) partial[(b * chunks + c) * (n + 1) + n] = sl;
-  }
-  if (!lam) return;
-  for (int bit = 0; bit < n; ++bit) {
-    double v;
-    if (bit < 8) {
-      v = ((threadIdx.x >> bit) & 1) ? -total : total;
-    } else if (bit < 12) {
-      v = 0.0;
+  // this thread's share of each low bit's signed sum (bits >= 12 are fixed
+  // per chunk: plus or minus the chunk's total), the total and the loss,
+  // reduced together once
+  constexpr int kVals = kLowBits + 2;
+  double vals[kVals];
 #pragma unroll
-      for (int k = 0; k < K; ++k) v += ((k >> (bit - 8)) & 1) ? -vk[k] : vk[k];
-    } else {
-      v = ((base >> bit) & 1) ? -total : total;
-    }
-    const double sum = block_sum(v, scratch);
-    if (threadIdx.x == 0) partial[(b * chunks + c) * (n + 1) + (n - 1 - bit)] = sum;
+  for (int bit = 0; bit < 8; ++bit) vals[bit] = ((threadIdx.x >> bit) & 1) ? -total : total;
+#pragma unroll
+  for (int bit = 8; bit < kLowBits; ++bit) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v += ((k >> (bit - 8)) & 1) ? -vk[k] : vk[k];
+    vals[bit] = v;
+  }
+  vals[kLowBits] = total;
+  vals[kLowBits + 1] = loss;
+  __shared__ double red[kBlock / 64][kVals];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < kVals; ++i) {
+    const double v = wave_sum(vals[i]);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kVals) {
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) r += red[w][threadIdx.x];
+    red[0][threadIdx.x] = r;  // each thread reads and writes only its own column
+  }
+  __syncthreads();
+  double* out = partial + (b * chunks + c) * (n + 1);
+  if (want_loss && threadIdx.x == 0) out[n] = red[0][kLowBits + 1];
+  if (!lam) return;
+  for (int bit = threadIdx.x; bit < n; bit += kBlock) {
+    const double sum = bit < kLowBits ? red[0][bit]
+                                      : (((base >> bit) & 1) ? -red[0][kLowBits]
+                                                             : red[0][kLowBits]);
+    out[n - 1 - bit] = sum;
   }
 }
 

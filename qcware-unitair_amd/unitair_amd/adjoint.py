@@ -125,8 +125,14 @@ def _rotation_table(spec, params, dtype, sign, repeat=1):
         if not keys:
             continue
         dev = params.device
-        t = (params[:, torch.tensor(idxs, device=dev)] *
-             torch.tensor(coefs, dtype=params.dtype, device=dev)).t()      # (R, B)
+        ckey = (spec.key, kind, sign, params.dtype, str(dev))
+        consts = _GROUP_INDEX.get(ckey) if spec.key is not None else None
+        if consts is None:       # device constants, made once per gate list (see _group_index)
+            consts = (torch.tensor(idxs, device=dev),
+                      torch.tensor(coefs, dtype=params.dtype, device=dev))
+            if spec.key is not None:
+                _GROUP_INDEX[ckey] = consts
+        t = (params[:, consts[0]] * consts[1]).t()                           # (R, B)
         data = _rotation_data(kind, t, dtype)                               # (R, B, ...)
         if repeat > 1:
             data = data.repeat((1, repeat) + (1,) * (data.dim() - 2))
@@ -165,6 +171,28 @@ def _gates_key(gates, n: int, num_params: int) -> int:
                 parts.append(("array", tuple(np.shape(x))))
         key.append(tuple(parts))
     return _KEY_IDS.setdefault((n, num_params, tuple(key)), len(_KEY_IDS))
+
+
+_GROUP_INDEX = {}
+
+
+def _group_index(spec, s, rot, dev):
+    """Device index tensors of rotation group s (Pauli kind, qubit,
+    parameter index, 2 * coefficient), made once per gate list and device:
+    building them per call would copy from the host, which waits for the
+    stream, once per group."""
+    key = (spec.key, s, str(dev))
+    hit = _GROUP_INDEX.get(key) if spec.key is not None else None
+    if hit is None:
+        hit = (torch.tensor([_PAULI[r[0]] for r in rot], device=dev),
+               torch.tensor([r[1] for r in rot], device=dev),
+               torch.tensor([r[2] for r in rot], device=dev),
+               torch.tensor([2.0 * r[3] for r in rot], dtype=torch.float64, device=dev))
+        if spec.key is not None:
+            if len(_GROUP_INDEX) >= 4 * _TEMPLATE_LIMIT:
+                _GROUP_INDEX.pop(next(iter(_GROUP_INDEX)))
+            _GROUP_INDEX[key] = hit
+    return hit
 
 
 def _program(items, n, dtype, device, compiled=False, key=None):
@@ -324,10 +352,7 @@ class _AdjointExpectation(torch.autograd.Function):
                     _lib.check(rc, "pauli_gradients")
                     last_traffic["backward_bytes"] += _bytes(X)
                 # one gather-add per group: grad[:, idx] += 2 coef vals[kind, :, q]
-                kinds = torch.tensor([_PAULI[r[0]] for r in rot], device=dev)
-                qs = torch.tensor([r[1] for r in rot], device=dev)
-                idx = torch.tensor([r[2] for r in rot], device=dev)
-                coef = torch.tensor([2.0 * r[3] for r in rot], dtype=torch.float64, device=dev)
+                kinds, qs, idx, coef = _group_index(spec, s, rot, dev)
                 grad.index_add_(1, idx, vals[kinds, :, qs].transpose(0, 1) * coef)
                 if progs.inv[s] is not None:
                     _run_counted(progs.inv[s], X, "backward_bytes")
@@ -374,6 +399,7 @@ def run_gates(params: torch.Tensor, gates: Sequence[tuple], num_qubits: int,
     obs = torch.zeros(2 ** n, dtype=torch.float64, device=params.device)
     _check_inputs(params, obs, n, dtype, params.device)
     spec = _Spec(gates, n, params.shape[1])
+    spec.key = _gates_key(gates, n, params.shape[1])
     B, dev = params.shape[0], params.device
     p = params.detach().to(_REAL[dtype])
     with on_device(dev):
@@ -382,8 +408,7 @@ def run_gates(params: torch.Tensor, gates: Sequence[tuple], num_qubits: int,
         for s, (fixed, rot) in enumerate(spec.segments):
             items += list(fixed) + _rot_items(s, rot, table, n)
         items += list(spec.trailing)
-        prog = _program(items, n, dtype, dev, compiled,
-                        (_gates_key(gates, n, params.shape[1]), "run"))
+        prog = _program(items, n, dtype, dev, compiled, (spec.key, "run"))
         psi = _initial(B, n, dtype, dev, initial_state)
         if prog is not None:
             prog.run(psi)
