@@ -604,7 +604,9 @@ class PlanTemplate:
 
 
 # structure key -> handle of its compiled pass (handles live as long as the
-# process; the library keeps the same table keyed by the generated source)
+# process; the library keeps the same table keyed by the generated source).
+# The key leaves out UA_TUNE_COMPILED_WAVES: set that knob before the first
+# compile of a structure.
 _COMPILED = {}
 
 
