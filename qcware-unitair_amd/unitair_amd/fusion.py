@@ -603,6 +603,58 @@ class PlanTemplate:
                         PlanTemplate.evaluate(recipe[2], items))
 
 
+def _structure_key(code: int, n: int, fp, host_ops: int) -> tuple:
+    """What a compiled pass depends on: dtype, n, the pass record and the kind
+    and bits of each op (not the data pointers or strides)."""
+    count = fp.stage_ops[fp.num_stages]
+    ops = (FusedOp * max(1, count)).from_address(host_ops)
+    return (code, n, bytes(fp), tuple((ops[j].kind, ops[j].a, ops[j].b) for j in range(count)))
+
+
+_DUMMY_DATA = {"dense1": np.eye(2), "dense2": np.eye(4), "diag1": np.ones(2),
+               "cz_layer": np.zeros(1, dtype=np.int64)}
+_BACKGROUND = []     # the executor of compile_in_background, made on first use
+
+
+def compile_in_background(items, n: int, dtype: torch.dtype = torch.complex64,
+                          reorder: bool = True) -> concurrent.futures.Future:
+    """Compile the passes of `items` (those with 12 tile bits) on a background
+    host thread, for FusedProgram(compiled=True) to pick up later without
+    compiling: the plan depends only on the items' kinds and bits, so it is
+    made from placeholder data and nothing touches the GPU.  The future's
+    result is the number of passes compiled."""
+    code = _lib.DTYPE_CODES[dtype]
+    shapes = [_Item(it.kind, tuple(it.bits), _DUMMY_DATA.get(it.kind), False) for it in items]
+
+    def job():
+        lib = _lib.load()
+        template = PlanTemplate(shapes, n, reorder=reorder)
+        done = 0
+        for st in template.steps:
+            if st[0] != "fused" or st[1].tile_bits != 12:
+                continue
+            _, fp, first, count = st
+            ops = (FusedOp * max(1, count))(*[FusedOp.from_buffer_copy(o)
+                                               for o in template.op_structs[first:first + count]])
+            for o in ops:
+                if o.kind not in (FOP_CZ, FOP_CNOT, FOP_LAYER1Q):
+                    o.data = 0x10000      # any aligned non-NULL address: nothing runs
+            key = _structure_key(code, n, fp, ctypes.addressof(ops))
+            if key in _COMPILED:
+                continue
+            handle = ctypes.c_void_p()
+            _lib.check(lib.ua_compile_fused_pass(code, n, ctypes.byref(fp), ctypes.addressof(ops),
+                                                 ctypes.byref(handle)), "compiling a fused pass")
+            _COMPILED[key] = handle
+            done += 1
+        return done
+
+    if not _BACKGROUND:
+        _BACKGROUND.append(concurrent.futures.ThreadPoolExecutor(
+            max_workers=max(1, min(8, (os.cpu_count() or 2) // 2))))
+    return _BACKGROUND[0].submit(job)
+
+
 # structure key -> handle of its compiled pass (handles live as long as the
 # process; the library keeps the same table keyed by the generated source).
 # The key leaves out UA_TUNE_COMPILED_WAVES: set that knob before the first
@@ -726,13 +778,7 @@ class FusedProgram:
             self._compile()
 
     def _structure_key(self, i):
-        """What a compiled pass depends on: dtype, n, the pass record and the
-        kind and bits of each op (not the data pointers or strides)."""
-        fp, host = self._calls[i][1], self._calls[i][3]
-        count = fp.stage_ops[fp.num_stages]
-        ops = (FusedOp * max(1, count)).from_address(host)
-        return (self.code, self.n, bytes(fp),
-                tuple((ops[j].kind, ops[j].a, ops[j].b) for j in range(count)))
+        return _structure_key(self.code, self.n, self._calls[i][1], self._calls[i][3])
 
     def _compile(self):
         """Replace every pass with 12 tile bits by its compiled kernel

@@ -54,10 +54,14 @@ import torch
 from . import _lib
 from ._device import on_device
 from ._hip import StreamGraph
-from .fusion import FusedProgram, _Item
+from .fusion import FusedProgram, _Item, compile_in_background
 from .gates import haar_unitary
 
 BYTES_C64 = 8
+# (qubit count, layers per step) -> future of the background compile of that
+# lane program's passes (their structure depends on nothing else); shared by
+# every lane and server of the process
+_BACKGROUND_JOBS: Dict[tuple, object] = {}
 
 
 @dataclass
@@ -183,20 +187,35 @@ class _Lane:
     def _program(self, K: int, used: int, buf: int):
         """FusedProgram (or its graph) for K layers over slots [0, used)
         reading staging buffer `buf`."""
-        key = (K, used, buf)
-        if key not in self.programs:
+        def layer_items():
             items = []
             for l in range(K):
                 for q in range(self.n):
                     items.append(_Item("dense1", (self.n - 1 - q,), self.mats[buf][l, q], True))
                 items.append(_Item("cz_layer", (), self.masks[buf][l], True))
+            return items
+
+        # compiled passes: always (compiled=True), or once a background
+        # compile of this K's structure has finished ("background": until
+        # then the lane runs the interpreted passes and never waits for it)
+        compiled = self.compiled is True
+        if self.compiled == "background" and self.n >= 12:
+            job = _BACKGROUND_JOBS.get((self.n, K))
+            if job is None:
+                _BACKGROUND_JOBS[(self.n, K)] = compile_in_background(
+                    layer_items(), self.n, torch.complex64, reorder=False)
+            else:
+                compiled = job.done() and job.exception() is None
+        key = (K, used, buf, compiled)
+        if key not in self.programs:
+            items = layer_items()
             # built on the default stream: its op-record upload waits only
             # for that stream, not for the lane's queued steps
             # in order: a request's gates meet each amplitude in the same
             # order whatever K and the other slots, so its result is the same
             # alone and in the mix
             prog = FusedProgram.from_items(items, self.n, torch.complex64, self.device,
-                                           reorder=False, compiled=self.compiled)
+                                           reorder=False, compiled=compiled)
             # per-slot data: consecutive slots are 4 (2x2) elements apart in
             # mats[buf][l, q] and 1 mask apart in masks[buf][l], the defaults
             view = self.states[:used]
@@ -277,15 +296,18 @@ class Server:
     HBM budget decide); max_active_amplitudes limits the sum of 2^n over the
     active requests (None: no limit); max_slots=1, max_active=1 runs one
     request at a time on the same fused programs.  compiled: lanes of 12 or
-    more qubits run compiled passes (FusedProgram(compiled=True)); the first
-    step of each (qubit count, layers per step) compiles its passes on the
-    host, about a second each, so this suits a steady mix of circuit shapes
-    (compiled passes are kept for the life of the process)."""
+    more qubits run compiled passes (FusedProgram(compiled=True)); with True
+    the first step of each (qubit count, layers per step) compiles its passes
+    on the host, about a second each, which suits a steady mix of circuit
+    shapes; with "background" that compile runs on a background host thread
+    and the lane keeps the interpreted passes until it is done, so no step
+    waits for it (compiled passes are kept for the life of the process).
+    Results agree between the two forms to rounding."""
 
     def __init__(self, hbm_budget: int = 64 << 30, max_slots: int = 32,
                  max_layers_per_step: int = 8, device="cuda", use_graphs: bool = True,
                  max_active: Optional[int] = None,
-                 max_active_amplitudes: Optional[int] = 1 << 22, compiled: bool = False):
+                 max_active_amplitudes: Optional[int] = 1 << 22, compiled=False):
         self.budget = hbm_budget
         self.compiled = compiled
         self.max_slots = max_slots
@@ -407,11 +429,13 @@ def run_one_at_a_time(requests: List[Request], device="cuda", realtime: bool = T
     return requests
 
 
-def run_one_at_a_time_fused(requests: List[Request], device="cuda", realtime: bool = True):
+def run_one_at_a_time_fused(requests: List[Request], device="cuda", realtime: bool = True,
+                            compiled=False):
     """One request at a time on the server's own fused programs (one slot,
     one request on the GPU): the server's gain over this is batching and
     overlap alone, not fusion."""
-    return Server(max_slots=1, max_active=1, device=device).run(requests, realtime=realtime)
+    return Server(max_slots=1, max_active=1, device=device, compiled=compiled).run(
+        requests, realtime=realtime)
 
 
 def summarize(requests: List[Request]):

@@ -116,3 +116,29 @@ def _pass_with_self_cz(n):
     op.kind, op.a, op.b = 3, 5, 5          # UA_FOP_CZ on state bits 5 and 5
     arr = (FusedOp * 1)(op)
     return fp, ctypes.addressof(arr), arr
+
+
+def test_compile_in_background_fills_the_table():
+    """compile_in_background plans with placeholder data and compiles on a
+    host thread (no GPU); the table then holds every 12-bit pass of the plan
+    made from the real items, under the keys FusedProgram looks up."""
+    import numpy as np
+    from unitair_amd import fusion
+    from unitair_amd.fusion import PlanTemplate, _Item, _structure_key
+    n = 13
+    rng = np.random.default_rng(0)
+    items = []
+    for layer in range(3):
+        for q in range(n):
+            items.append(_Item("dense1", (n - 1 - q,), rng.standard_normal((2, 2)), False))
+        items.append(_Item("cz_layer", (), np.array([0b1010101], dtype=np.int64), False))
+    done = fusion.compile_in_background(items, n, reorder=False).result(timeout=300)
+    assert done >= 1
+    t = PlanTemplate(items, n, reorder=False)
+    fused = [st for st in t.steps if st[0] == "fused" and st[1].tile_bits == 12]
+    assert fused
+    for _, fp, first, count in fused:
+        arr = (FusedOp * count)(*t.op_structs[first:first + count])
+        assert _structure_key(_lib.UA_COMPLEX64, n, fp, ctypes.addressof(arr)) in fusion._COMPILED
+    # a second request for the same structure compiles nothing
+    assert fusion.compile_in_background(items, n, reorder=False).result(timeout=300) == 0

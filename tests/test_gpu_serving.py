@@ -110,3 +110,29 @@ def test_compiled_lanes_match_oracle(device):
         norm, z0 = oracle_answer(r)
         assert abs(r.norm2 - 1.0) < 1e-5
         assert abs(r.z0 - z0) < 1e-5, (r.n, r.depth)
+
+
+def test_background_compiled_lanes_match_oracle(device):
+    """Server(compiled="background"): lanes start on interpreted passes while
+    their structure compiles on a host thread, then switch; every result
+    against the complex128 oracle."""
+    from unitair_amd.serving import _BACKGROUND_JOBS
+    reqs = [Request(i, n, d, seed=700 + i) for i, (n, d) in
+            enumerate([(13, 30), (15, 30), (13, 24), (12, 40)])]
+    Server(device=device, max_layers_per_step=3, compiled="background").run(reqs,
+                                                                          realtime=False)
+    for r in reqs:
+        norm, z0 = oracle_answer(r)
+        assert abs(r.norm2 - 1.0) < 1e-5
+        assert abs(r.z0 - z0) < 1e-5, (r.n, r.depth)
+    jobs = [j for (n, k), j in _BACKGROUND_JOBS.items() if n in (12, 13, 15)]
+    assert jobs
+    for j in jobs:
+        j.result(timeout=120)            # the compiles themselves succeed
+    # a second run of the same shapes starts on compiled passes
+    again = [Request(10 + i, n, d, seed=800 + i) for i, (n, d) in enumerate([(13, 9), (15, 6)])]
+    Server(device=device, max_layers_per_step=3, compiled="background").run(again,
+                                                                          realtime=False)
+    for r in again:
+        norm, z0 = oracle_answer(r)
+        assert abs(r.z0 - z0) < 1e-5, (r.n, r.depth)
