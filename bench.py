@@ -393,26 +393,6 @@ def serving_bench(dev, num=200, seed=11, cap_sample=40):
                                        realtime=True)
     fused1_hi = run_one_at_a_time_fused(poisson_workload(num, rate=rate_hi, seed=seed),
                                         device=dev, realtime=True)
-    # lanes of 12+ qubits on compiled passes, compiled on background host
-    # threads while the lanes run on interpreted ones: saturated runs first
-    # (they start the compiles), then the heavy load once every compile
-    # they started has finished
-    from unitair_amd.serving import _BACKGROUND_JOBS
-    log("serving: compiled passes (background compiles)")
-    t_c = time.perf_counter()
-    sat_c = poisson_workload(num, rate=0.0, seed=seed)
-    Server(device=dev, compiled="background").run(sat_c, realtime=False)
-    capacity_cb_c = num / max(r.done for r in sat_c)     # while the compiles run
-    for job in list(_BACKGROUND_JOBS.values()):
-        job.result()
-    compile_wait = time.perf_counter() - t_c
-    sat_c1 = poisson_workload(num, rate=0.0, seed=seed)
-    run_one_at_a_time_fused(sat_c1, device=dev, realtime=False, compiled="background")
-    capacity_f1_c = num / max(r.done for r in sat_c1)
-    served_c_hi = Server(device=dev, compiled="background").run(
-        poisson_workload(num, rate=rate_hi, seed=seed), realtime=True)
-    fused1_c_hi = run_one_at_a_time_fused(poisson_workload(num, rate=rate_hi, seed=seed),
-                                          device=dev, realtime=True, compiled="background")
     same_fused = max(max(abs(a.z0 - b.z0), abs(a.norm2 - b.norm2))
                      for a, b in zip(served, fused1))
     same_plain = max(max(abs(a.z0 - b.z0), abs(a.norm2 - b.norm2))
@@ -432,21 +412,6 @@ def serving_bench(dev, num=200, seed=11, cap_sample=40):
             "heavy_load": {"arrival_rate_per_s": rate_hi,
                            "continuous_batching": summarize(served_hi),
                            "one_at_a_time_fused_programs": summarize(fused1_hi)},
-            "compiled_background": {
-                "note": "lanes of 12+ qubits on compiled passes, compiled on background "
-                        "host threads during a saturated continuous-batching run; the "
-                        "one-at-a-time capacity and the heavy-load runs come after those "
-                        "compiles finished",
-                "structures": len(_BACKGROUND_JOBS),
-                "saturated_run_and_compile_wait_s": compile_wait,
-                "continuous_batching_capacity_while_compiling_circuits_per_s": capacity_cb_c,
-                "one_at_a_time_fused_capacity_circuits_per_s": capacity_f1_c,
-                "heavy_load": {"arrival_rate_per_s": rate_hi,
-                               "continuous_batching": summarize(served_c_hi),
-                               "one_at_a_time_fused_programs": summarize(fused1_c_hi)},
-                "max_difference_vs_interpreted": max(
-                    max(abs(a.z0 - b.z0), abs(a.norm2 - b.norm2))
-                    for a, b in zip(served_c_hi, served_hi))},
             "max_difference_vs_fused_alone": same_fused,
             "max_difference_vs_per_gate": same_plain}
 
@@ -857,13 +822,7 @@ def main():
         log("serving workload")
         extra["serving"] = serving_bench(dev)
         sv = extra["serving"]
-        cb = sv["compiled_background"]
-        log(f"serving compiled: one-at-a-time capacity "
-            f"{cb['one_at_a_time_fused_capacity_circuits_per_s']:.1f} circ/s "
-            f"({cb['structures']} structures compiled during a saturated run, "
-            f"{cb['saturated_run_and_compile_wait_s']:.1f} s)")
-        for tag, blk in (("", sv), ("heavy load ", sv["heavy_load"]),
-                         ("compiled heavy load ", cb["heavy_load"])):
+        for tag, blk in (("", sv), ("heavy load ", sv["heavy_load"])):
             for key in ("continuous_batching", "one_at_a_time_fused_programs",
                         "one_at_a_time_per_gate"):
                 if key in blk:
