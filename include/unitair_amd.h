@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define UA_ABI_VERSION 1
+#define UA_ABI_VERSION 2
 
 /* Element types (torch.float32, float64, complex64, complex128). */
 enum {
@@ -387,6 +387,7 @@ int ua_run_dense_blocks(void* state, int dtype, int num_qubits, int64_t batch,
  * PHASE and PHASE_FACTORS ops belong in diagonal stages.
  */
 #define UA_FUSED_MAX_STAGES 32
+#define UA_FUSED_MAX_TILE_BITS 16
 
 enum {
   UA_FOP_DENSE1 = 0, /* a: cube bit; data: 2x2 matrix (state dtype)          */
@@ -421,8 +422,10 @@ typedef struct {
 } ua_fused_op;
 
 typedef struct {
-  int32_t tile_bits;           /* m in 4..12, m <= num_qubits */
-  int32_t local_bits[12];      /* state bit of tile bit i: ascending, [0..4) = 0,1,2,3 */
+  int32_t tile_bits;           /* m <= num_qubits: 4..12 for ua_run_fused_pass; compiled
+                                  passes 12..14 (complex64) or 12..13 (complex128) */
+  int32_t local_bits[UA_FUSED_MAX_TILE_BITS]; /* state bit of tile bit i (i < m):
+                                  ascending, [0..4) = 0,1,2,3 */
   int32_t num_stages;          /* <= UA_FUSED_MAX_STAGES */
   int32_t needs_index;         /* unused (derived from the ops); kept for the layout */
   int32_t stage_bits[UA_FUSED_MAX_STAGES][4]; /* ascending tile bits of each cube, or
@@ -452,8 +455,10 @@ int ua_run_fused_pass(void* state, int dtype, int num_qubits, int64_t batch,
  * valid until the process ends (each GPU loads the code the first time the
  * pass runs there).  Compilation takes of the order of a second and runs on
  * the host only; calls on different threads compile in parallel.
- * Requires tile_bits == 12 (UA_ERR_UNSUPPORTED otherwise); every argument
- * check of ua_run_fused_pass applies.
+ * Tiles of 12..14 bits for complex64 (a 2^14-amplitude tile is 128 KiB of
+ * LDS, 1024 threads) and 12..13 for complex128 (UA_ERR_UNSUPPORTED
+ * otherwise): larger tiles hold more gates per pass, so a circuit needs
+ * fewer passes.  Every other argument check of ua_run_fused_pass applies.
  */
 int ua_compile_fused_pass(int dtype, int num_qubits, const ua_fused_pass* pass,
                           const ua_fused_op* host_ops, void** handle);

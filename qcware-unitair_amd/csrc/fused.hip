@@ -715,13 +715,14 @@ bool is_diag_kind(int kind) {
 }
 
 int check_fused_pass(int dtype, int n, const ua_fused_pass* pass, const ua_fused_op* host_ops,
-                     FusedPassInfo* info) {
+                     FusedPassInfo* info, int max_tile_bits) {
   UA_CHECK_ARG(dtype == UA_COMPLEX64 || dtype == UA_COMPLEX128,
                "fused passes run on complex64/complex128 states");
   UA_CHECK_ARG(pass != nullptr, "NULL pass");
   const int m = pass->tile_bits;
   UA_CHECK_ARG(n >= 4 && n <= 62, "fused passes need 4..62 qubits (got %d)", n);
-  UA_CHECK_ARG(m >= 4 && m <= kTileBits && m <= n, "tile_bits=%d invalid for %d qubits", m, n);
+  UA_CHECK_ARG(m >= 4 && m <= max_tile_bits && m <= n, "tile_bits=%d invalid for %d qubits", m,
+               n);
   UA_CHECK_ARG(pass->num_stages >= 0 && pass->num_stages <= UA_FUSED_MAX_STAGES,
                "num_stages=%d out of range", pass->num_stages);
   UA_CHECK_ARG(host_ops != nullptr || pass->num_stages == 0,
@@ -823,7 +824,7 @@ int check_fused_pass(int dtype, int n, const ua_fused_pass* pass, const ua_fused
 int fused_pass_impl(void* state, int dtype, int n, int64_t batch, const ua_fused_pass* pass,
                     const ua_fused_op* ops, const ua_fused_op* host_ops, hipStream_t s) {
   FusedPassInfo info{};
-  const int rc = check_fused_pass(dtype, n, pass, host_ops, &info);
+  const int rc = check_fused_pass(dtype, n, pass, host_ops, &info, kTileBits);
   if (rc != UA_OK) return rc;
   UA_CHECK_ARG(batch >= 0, "negative batch");
   if (batch == 0) return UA_OK;

@@ -16,11 +16,11 @@ struct FusedPassInfo {
 
 bool is_diag_kind(int kind);
 
-// Checks dtype, qubit count, tile bits, stages and every op of `pass`
-// against host_ops; fills *info.  Returns UA_OK or UA_ERR_INVALID_ARGUMENT
-// (with the message set).
+// Checks dtype, qubit count, tile bits (4..max_tile_bits), stages and every
+// op of `pass` against host_ops; fills *info.  Returns UA_OK or
+// UA_ERR_INVALID_ARGUMENT (with the message set).
 int check_fused_pass(int dtype, int n, const ua_fused_pass* pass, const ua_fused_op* host_ops,
-                     FusedPassInfo* info);
+                     FusedPassInfo* info, int max_tile_bits);
 
 // UA_TUNE_COMPILED_WAVES (fused_jit.hip).
 int set_compiled_waves(int waves);

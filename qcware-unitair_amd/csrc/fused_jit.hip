@@ -35,7 +35,9 @@
 namespace ua {
 namespace {
 
-constexpr int kTileBits = 12;
+constexpr int kMinTileBits = 12;  // tiles below 12 bits stay with the interpreter
+// largest tile: 2^14 complex64 amplitudes (128 KiB of LDS) or 2^13 complex128
+int max_tile_bits(int dtype) { return dtype == UA_COMPLEX128 ? 13 : 14; }
 
 // Waves per SIMD the compiled complex64 kernels ask the register allocator
 // for (0: 4, the default; a complex64 tile takes 32 KiB of LDS, so at most
@@ -305,33 +307,40 @@ std::string generate(const PassShape& ps) {
   const int* local = p.local_bits;
   std::ostringstream s;
   const bool dbl = ps.dtype == UA_COMPLEX128;
+  const int tbits = p.tile_bits;              // 12..14: 2^(tbits-4) threads of 16 amplitudes
+  const int threads = 1 << (tbits - 4);
   s << "#define UA_DOUBLE " << (dbl ? 1 : 0) << "\n" << kPrelude;
   const int waves = dbl ? 2 : (g_compiled_waves > 0 ? int(g_compiled_waves) : 4);
-  s << "extern \"C\" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu("
+  s << "extern \"C\" __global__ __launch_bounds__(" << threads
+    << ") __attribute__((amdgpu_waves_per_eu("
     << waves << ")))\nvoid ua_compiled_pass(T* __restrict__ state, "
     << "const OpRec* __restrict__ ops_arg) {\n";
   s << "  const SC OpRec* ops = (const SC OpRec*)ops_arg;\n"
     << "  const int tid = threadIdx.x;\n"
     << "  const u64 tg = blockIdx.x;\n"
-    << "  const long long b = (long long)(tg >> " << (n - kTileBits) << ");\n"
-    << "  u64 origin = tg & " << hex64((uint64_t(1) << (n - kTileBits)) - 1) << ";\n";
+    << "  const long long b = (long long)(tg >> " << (n - tbits) << ");\n"
+    << "  u64 origin = tg & " << hex64((uint64_t(1) << (n - tbits)) - 1) << ";\n";
   // tile origin: the tile index with zero bits inserted at the local bits
-  for (int i = 0; i < kTileBits; ++i)
+  for (int i = 0; i < tbits; ++i)
     s << "  origin = ((origin >> " << local[i] << ") << " << local[i] + 1 << ") | (origin & "
       << hex64((uint64_t(1) << local[i]) - 1) << ");\n";
   s << "  GL T* const st = (GL T*)state + b * " << hex64(uint64_t(1) << n) << ";\n"
     << "  const int t2 = 2 * tid;\n"
     << "  const u64 dep_t = u64(t2 & 15)";
-  for (int i = 4; i < 9; ++i) s << " | (u64((t2 >> " << i << ") & 1) << " << local[i] << ")";
+  for (int i = 4; i < tbits - 3; ++i)
+    s << " | (u64((t2 >> " << i << ") & 1) << " << local[i] << ")";
   s << ";\n  const int lds_t = lds_pos(t2);\n";
-  // pair k of this thread: state index origin | depk[k] | dep_t, LDS
-  // position lds_t ^ ldsk[k] (the wave-uniform part of 2t + 512k)
+  // pair k of this thread (tile element 2t + 2 * threads * k: bit 0 the
+  // pair, bits 1..tbits-4 the thread, the top 3 bits k): state index
+  // origin | depk[k] | dep_t, LDS position lds_t ^ ldsk[k]
   uint64_t depk[8];
   int ldsk[8];
   for (int k = 0; k < 8; ++k) {
     depk[k] = 0;
-    for (int i = 9; i < kTileBits; ++i) depk[k] |= uint64_t((k >> (i - 9)) & 1) << local[i];
-    ldsk[k] = (512 * k) ^ (16 * (k & 1));
+    for (int i = tbits - 3; i < tbits; ++i)
+      depk[k] |= uint64_t((k >> (i - (tbits - 3))) & 1) << local[i];
+    const int e = 2 * threads * k;
+    ldsk[k] = e ^ ((e >> 5) & 31);  // lds_pos, linear over GF(2)
   }
   auto gd = [&](int o) {
     std::ostringstream e;
@@ -446,7 +455,7 @@ std::string generate(const PassShape& ps) {
     auto bit_of = [&](int bit) {
       std::ostringstream e;
       int tile_pos = -1;
-      for (int i = 0; i < kTileBits; ++i)
+      for (int i = 0; i < tbits; ++i)
         if (local[i] == bit) tile_pos = i;
       if (tile_pos >= 0)
         e << "(((e0 >> " << tile_pos << ") & 1) != 0)";
@@ -468,7 +477,7 @@ std::string generate(const PassShape& ps) {
       << ">;\n    const int e0 = C::e0(tid);\n    T q[16];\n";
     if (need_gb) {  // state index of cube entry 0
       s << "    const u64 gb = origin";
-      for (int i = 0; i < kTileBits; ++i)
+      for (int i = 0; i < tbits; ++i)
         s << " | (u64((e0 >> " << i << ") & 1) << " << local[i] << ")";
       s << ";\n";
     }
@@ -549,7 +558,8 @@ std::string generate(const PassShape& ps) {
   };
 
   if (uses_lds)
-    s << "  __shared__ T tile[4096];\n  char* const tb = reinterpret_cast<char*>(tile);\n";
+    s << "  __shared__ T tile[" << (1 << tbits)
+      << "];\n  char* const tb = reinterpret_cast<char*>(tile);\n";
   if (uses_lds && lds_first < lds_end && has_factors(lds_first)) emit_prefetch(lds_first);
   else if (uses_lds && lds_first >= lds_end && !last_direct && tail && has_factors(ns - 1))
     emit_prefetch(ns - 1);
@@ -610,7 +620,7 @@ std::string generate(const PassShape& ps) {
 
 // A compiled pass: its code object, and the function loaded on each device.
 struct Compiled {
-  int dtype = 0, n = 0;
+  int dtype = 0, n = 0, tile_bits = 0;
   std::vector<char> code;
   std::mutex mu;
   std::map<int, hipFunction_t> fn;
@@ -625,10 +635,11 @@ std::map<std::string, std::unique_ptr<Compiled>>& cache() {
 int check_shape(int dtype, int n, const ua_fused_pass* pass, const ua_fused_op* host_ops,
                 PassShape* ps) {
   FusedPassInfo info{};
-  const int rc = check_fused_pass(dtype, n, pass, host_ops, &info);
+  const int rc = check_fused_pass(dtype, n, pass, host_ops, &info, max_tile_bits(dtype));
   if (rc != UA_OK) return rc;
-  if (pass->tile_bits != kTileBits) {
-    set_error("compiled passes need tile_bits == 12 (got %d)", pass->tile_bits);
+  if (pass->tile_bits < kMinTileBits) {
+    set_error("compiled passes need tile_bits of 12..%d (got %d)", max_tile_bits(dtype),
+              pass->tile_bits);
     return UA_ERR_UNSUPPORTED;
   }
   *ps = PassShape{dtype, n, pass, host_ops, info};
@@ -713,6 +724,7 @@ extern "C" int ua_compile_fused_pass(int dtype, int num_qubits, const ua_fused_p
   auto c = std::make_unique<ua::Compiled>();
   c->dtype = dtype;
   c->n = num_qubits;
+  c->tile_bits = pass->tile_bits;
   rc = ua::compile(src, &c->code);  // outside the lock: compiles run in parallel
   if (rc != UA_OK) return rc;
   std::lock_guard<std::mutex> lock(ua::g_cache_mu);
@@ -767,13 +779,14 @@ extern "C" int ua_run_compiled_pass(void* handle, void* state, int64_t batch,
   UA_CHECK_ARG(state != nullptr && ops != nullptr, "NULL pointer");
   UA_CHECK_ARG(reinterpret_cast<uintptr_t>(state) % (2 * ua::elem_size(c->dtype)) == 0,
                "state must be aligned to two elements");
-  const int64_t blocks = batch << (c->n - ua::kTileBits);
-  UA_CHECK_ARG(batch <= (int64_t(0x7fffffff) >> (c->n - ua::kTileBits)), "too many tiles");
+  const int64_t blocks = batch << (c->n - c->tile_bits);
+  UA_CHECK_ARG(batch <= (int64_t(0x7fffffff) >> (c->n - c->tile_bits)), "too many tiles");
   hipFunction_t f = nullptr;
   const int rc = ua::function_of(c, &f);
   if (rc != UA_OK) return rc;
   void* args[] = {&state, &ops};
-  const hipError_t e = hipModuleLaunchKernel(f, unsigned(blocks), 1, 1, 256, 1, 1, 0,
+  const unsigned threads = 1u << (c->tile_bits - 4);
+  const hipError_t e = hipModuleLaunchKernel(f, unsigned(blocks), 1, 1, threads, 1, 1, 0,
                                              static_cast<hipStream_t>(stream), args, nullptr);
   if (e != hipSuccess) {
     ua::set_error("launching a compiled fused pass: %s", hipGetErrorString(e));

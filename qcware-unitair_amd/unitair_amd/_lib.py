@@ -21,7 +21,7 @@ UA_OK = 0
 UA_ERR_INVALID_ARGUMENT = 1
 UA_ERR_UNSUPPORTED = 2
 UA_ERR_HIP = 3
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 UA_FLOAT32 = 0
 UA_FLOAT64 = 1
@@ -103,6 +103,7 @@ class Gate(ctypes.Structure):
 
 
 UA_FUSED_MAX_STAGES = 32
+UA_FUSED_MAX_TILE_BITS = 16
 
 
 class FusedOp(ctypes.Structure):
@@ -114,7 +115,8 @@ class FusedOp(ctypes.Structure):
 
 class FusedPass(ctypes.Structure):
     """ua_fused_pass (include/unitair_amd.h)."""
-    _fields_ = [("tile_bits", ctypes.c_int32), ("local_bits", ctypes.c_int32 * 12),
+    _fields_ = [("tile_bits", ctypes.c_int32),
+                ("local_bits", ctypes.c_int32 * UA_FUSED_MAX_TILE_BITS),
                 ("num_stages", ctypes.c_int32), ("needs_index", ctypes.c_int32),
                 ("stage_bits", (ctypes.c_int32 * 4) * UA_FUSED_MAX_STAGES),
                 ("stage_ops", ctypes.c_int32 * (UA_FUSED_MAX_STAGES + 1))]

@@ -611,6 +611,12 @@ def _structure_key(code: int, n: int, fp, host_ops: int) -> tuple:
     return (code, n, bytes(fp), tuple((ops[j].kind, ops[j].a, ops[j].b) for j in range(count)))
 
 
+# default tiles of compiled passes: 2^13 amplitudes (64 KiB of LDS for
+# complex64, 128 KiB for complex128).  ua_compile_fused_pass also takes
+# 2^14-amplitude complex64 tiles (fewer passes, but measured slower on the
+# 24- and 28-qubit circuits: 1024-thread workgroups, one per CU).
+COMPILED_TILE_BITS = {torch.complex64: 13, torch.complex128: 13}
+
 _DUMMY_DATA = {"dense1": np.eye(2), "dense2": np.eye(4), "diag1": np.ones(2),
                "cz_layer": np.zeros(1, dtype=np.int64)}
 _BACKGROUND = []     # the executor of compile_in_background, made on first use
@@ -631,7 +637,7 @@ def compile_in_background(items, n: int, dtype: torch.dtype = torch.complex64,
         template = PlanTemplate(shapes, n, reorder=reorder)
         done = 0
         for st in template.steps:
-            if st[0] != "fused" or st[1].tile_bits != 12:
+            if st[0] != "fused" or st[1].tile_bits < TILE_BITS:
                 continue
             _, fp, first, count = st
             ops = (FusedOp * max(1, count))(*[FusedOp.from_buffer_copy(o)
@@ -669,7 +675,7 @@ class FusedProgram:
     be per state (a device tensor with a leading batch dimension)."""
 
     def __init__(self, circuit: Circuit, dtype: torch.dtype = torch.complex64,
-                 device="cuda", tile_bits: int = TILE_BITS, merge_1q: bool = True,
+                 device="cuda", tile_bits: Optional[int] = None, merge_1q: bool = True,
                  reorder: bool = True, compiled: bool = False, phase_angles: bool = False):
         items = _items_from_circuit(circuit)
         if merge_1q:
@@ -680,7 +686,7 @@ class FusedProgram:
 
     @classmethod
     def from_items(cls, items: List[_Item], n: int, dtype: torch.dtype = torch.complex64,
-                   device="cuda", tile_bits: int = TILE_BITS,
+                   device="cuda", tile_bits: Optional[int] = None,
                    reorder: bool = True, compiled: bool = False,
                    phase_angles: bool = False,
                    template: Optional[PlanTemplate] = None) -> "FusedProgram":
@@ -695,6 +701,9 @@ class FusedProgram:
 
     def _build(self, items, n, dtype, device, tile_bits, reorder=True, compiled=False,
                phase_angles=False, template=None):
+        if tile_bits is None:
+            # compiled passes take larger tiles (fewer passes); the interpreter 12
+            tile_bits = COMPILED_TILE_BITS.get(dtype, TILE_BITS) if compiled else TILE_BITS
         if dtype not in (torch.complex64, torch.complex128):
             raise ValueError("fused programs run complex64/complex128 states")
         for it in items:
@@ -781,10 +790,11 @@ class FusedProgram:
         return _structure_key(self.code, self.n, self._calls[i][1], self._calls[i][3])
 
     def _compile(self):
-        """Replace every pass with 12 tile bits by its compiled kernel
+        """Replace every pass with 12 or more tile bits by its compiled kernel
         (ua_compile_fused_pass; new structures compile on parallel threads,
         known ones come from a per-process table without a library call)."""
-        todo = [i for i, c in enumerate(self._calls) if c[0] == "fused" and c[1].tile_bits == 12]
+        todo = [i for i, c in enumerate(self._calls)
+                if c[0] == "fused" and c[1].tile_bits >= TILE_BITS]
         keys = {i: self._structure_key(i) for i in todo}
         new = [i for i in todo if keys[i] not in _COMPILED]
 

@@ -54,7 +54,7 @@ import torch
 from . import _lib
 from ._device import on_device
 from ._hip import StreamGraph
-from .fusion import FusedProgram, _Item, compile_in_background
+from .fusion import TILE_BITS, FusedProgram, _Item, compile_in_background
 from .gates import haar_unitary
 
 BYTES_C64 = 8
@@ -215,7 +215,8 @@ class _Lane:
             # order whatever K and the other slots, so its result is the same
             # alone and in the mix
             prog = FusedProgram.from_items(items, self.n, torch.complex64, self.device,
-                                           reorder=False, compiled=compiled)
+                                           tile_bits=TILE_BITS, reorder=False,
+                                           compiled=compiled)
             # per-slot data: consecutive slots are 4 (2x2) elements apart in
             # mats[buf][l, q] and 1 mask apart in masks[buf][l], the defaults
             view = self.states[:used]

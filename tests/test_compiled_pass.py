@@ -15,8 +15,8 @@ from unitair_amd.workloads import random_circuit_24q
 from test_fusion_plan import random_mixed_circuit
 
 
-def _structs(circ, n):
-    steps, ops, pending = _pass_structs(_merge_1q(_items_from_circuit(circ)), n)
+def _structs(circ, n, tile_bits=12):
+    steps, ops, pending = _pass_structs(_merge_1q(_items_from_circuit(circ)), n, tile_bits)
     for idx, _ in pending:
         ops[idx].data = 0x10000        # any aligned non-NULL address: nothing runs
     arr = (FusedOp * max(1, len(ops)))(*ops)
@@ -32,12 +32,13 @@ def _compile(lib, code, n, fp, host):
     return rc, h.value
 
 
-@pytest.mark.parametrize("code", [_lib.UA_COMPLEX64, _lib.UA_COMPLEX128])
-def test_every_pass_of_the_24q_circuit_compiles(code):
+@pytest.mark.parametrize("code,tile_bits", [(_lib.UA_COMPLEX64, 12), (_lib.UA_COMPLEX128, 12),
+                                           (_lib.UA_COMPLEX64, 14), (_lib.UA_COMPLEX128, 13)])
+def test_every_pass_of_the_24q_circuit_compiles(code, tile_bits):
     lib = _lib.load()
     n = 24
-    passes, arr = _structs(random_circuit_24q(seed=0, n=n), n)
-    assert len(passes) >= 10
+    passes, arr = _structs(random_circuit_24q(seed=0, n=n), n, tile_bits)
+    assert len(passes) >= 8 and all(fp.tile_bits == tile_bits for fp, _ in passes)
     with concurrent.futures.ThreadPoolExecutor(max_workers=8) as ex:
         res = list(ex.map(lambda p: _compile(lib, code, n, *p), passes))
     for rc, h in res:
@@ -81,8 +82,13 @@ def test_argument_checks():
     fp, host = passes[0]
     rc, h = _compile(lib, _lib.UA_COMPLEX64, n, fp, host)
     assert rc == _lib.UA_ERR_UNSUPPORTED and not h
-    with pytest.raises(NotImplementedError, match="tile_bits == 12"):
+    with pytest.raises(NotImplementedError, match="tile_bits of 12..14"):
         _lib.check(rc, "compile")
+    # above the largest tile: 14 bits for complex128
+    n = 16
+    passes, arr16 = _structs(random_mixed_circuit(n, 30, seed=4), n, 14)
+    rc, _ = _compile(lib, _lib.UA_COMPLEX128, n, *passes[0])
+    assert rc == _lib.UA_ERR_INVALID_ARGUMENT
     # the interpreter's validation applies (here: a float dtype)
     n = 14
     passes, arr = _structs(random_mixed_circuit(n, 20, seed=2), n)

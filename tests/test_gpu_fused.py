@@ -185,3 +185,19 @@ def test_compiled_phase_angles_accuracy(device):
     out = prog.run(torch.ones(1, 2 ** n, dtype=torch.complex64, device=device))
     err = np.abs(out.cpu().numpy()[0] - np.exp(-1j * ang.astype(np.float64)))
     assert np.max(err) < 3e-7, (int(np.argmax(err)), float(np.max(err)))
+
+
+@pytest.mark.parametrize("n,tile_bits", [(16, 12), (16, 14), (17, 13)])
+def test_compiled_tile_sizes(device, n, tile_bits):
+    """Compiled passes on 12-, 13- and 14-bit complex64 tiles against the oracle."""
+    circ = random_mixed_circuit(n, 90, seed=n + tile_bits)
+    prog = FusedProgram(circ, dtype=torch.complex64, device=device, compiled=True,
+                        tile_bits=tile_bits)
+    fused = [st for st in prog.steps if st[0] == "fused"]
+    assert fused and all(st[1].tile_bits == tile_bits for st in fused)
+    assert prog.num_compiled == len(fused)
+    psi = seeded_state_np(n, (2,), seed=9)
+    ref = np.stack([oc.run_closed_form(circ.gates, p) for p in psi])
+    out = prog.run(torch.from_numpy(psi).to(device, torch.complex64)).cpu().numpy()
+    assert rel_err(out, ref) < 1e-4
+    assert np.max(U.fidelity_error(ref, out)) < 1e-5
