@@ -206,7 +206,10 @@ def _program(items, n, dtype, device, compiled=False, key=None):
             if len(_TEMPLATES) >= _TEMPLATE_LIMIT:
                 _TEMPLATES.pop(next(iter(_TEMPLATES)))
             _TEMPLATES[key] = template
-    return FusedProgram.from_items(items, n, dtype, device, compiled=compiled,
+    # 12-bit tiles also for compiled passes: measured faster here than the
+    # 13-bit default of compiled programs (34.5 vs 38.4 ms per step at 16
+    # qubits x 1024 states)
+    return FusedProgram.from_items(items, n, dtype, device, tile_bits=12, compiled=compiled,
                                    template=template)
 
 
