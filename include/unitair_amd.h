@@ -43,7 +43,7 @@ enum {
 enum {
   UA_OK = 0,
   UA_ERR_INVALID_ARGUMENT = 1, /* shapes, qubits, dtypes or sizes rejected */
-  UA_ERR_UNSUPPORTED = 2,      /* valid request this build does not handle */
+  UA_ERR_UNSUPPORTED = 2,      /* valid request the library does not handle */
   UA_ERR_HIP = 3               /* a HIP runtime call or launch failed */
 };
 

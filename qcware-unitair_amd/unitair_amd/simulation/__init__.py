@@ -294,7 +294,7 @@ def apply_all_qubits(operator: torch.Tensor, state: torch.Tensor) -> torch.Tenso
     sb, expand_state, selectors, n_sel, op_batch, _ = _plan_batch(obatch, sbatch, n, 1, [0])
     if selectors:
         # The reference lines such dims up with qubit dims of a state whose
-        # qubits it keeps swapping; this build rejects them.
+        # qubits it keeps swapping; they are rejected here.
         raise ValueError(
             f"apply_all_qubits: operator batch {obatch} has dims beyond the state "
             f"batch {sbatch} that would be matched against qubit dims")
