@@ -514,7 +514,7 @@ def _ops_inputs(n, seed, device, dtype):
     psi = seeded_state(n, (), seed=seed, dtype=dtype, device=device)
     other = seeded_state(n, (), seed=seed + 1, dtype=dtype, device=device)
     u = {k: torch.from_numpy(haar_unitary(2 ** k, rng)).to(device, dtype)
-         for k in (1, 2, 5, 6, 8)}
+         for k in (1, 2, 5, 6, 7, 8)}
     ang = torch.from_numpy(rng.uniform(0, 2 * np.pi, 2 ** n).astype(np.float32)).to(device)
     diag = torch.from_numpy(rng.standard_normal(2 ** n).astype(np.float32)).to(device)
     return psi, other, u, ang, diag
@@ -548,7 +548,9 @@ def _op_table(n, sim, states, psi, other, u, ang, diag, sample, rand_state):
          lambda: sim.apply_operator(u[5], [0, 1, 2, 3, 4], psi), 2 * A * s),
         ("apply_operator 6q Haar, qubits (0, 7, 12, 3, 23, 18) (matrix cores, one-gate pass)",
          lambda: sim.apply_operator(u[6], [0, 7, 12, 3, 23, 18], psi), 2 * A * s),
-        ("apply_operator 8q Haar, qubits (2, 9, 14, 0, 21, 5, 17, 11) (tiled wide-gate kernel)",
+        ("apply_operator 7q Haar, qubits (2, 9, 14, 0, 21, 5, 17) (matrix cores, 2 row slices)",
+         lambda: sim.apply_operator(u[7], [2, 9, 14, 0, 21, 5, 17], psi), 2 * A * s),
+        ("apply_operator 8q Haar, qubits (2, 9, 14, 0, 21, 5, 17, 11) (matrix cores, 8 row slices)",
          lambda: sim.apply_operator(u[8], [2, 9, 14, 0, 21, 5, 17, 11], psi), 2 * A * s),
         ("act_first_qubits 2q Haar",
          lambda: sim.act_first_qubits(u[2], psi), 2 * A * s),

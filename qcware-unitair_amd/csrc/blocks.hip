@@ -24,7 +24,9 @@
 // fits (10 bits + 2).  Gates of other kinds or sizes, and states below 14
 // qubits, run one by one as ua_run_circuit runs them.  The same kernel, with
 // a 64 KB A-operand block, takes single 6-qubit complex64 gates of
-// apply_operator (in place or not).
+// apply_operator (in place or not); single 7- and 8-qubit complex64 gates
+// (out of place) run on dense_slice_kernel, which keeps one row slice of the
+// gate's real block in LDS.
 #include "mfma_common.h"
 
 namespace ua {
@@ -281,6 +283,272 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FRAG ==
   }
 }
 
+// ---- single 7- and 8-qubit complex64 gates on the matrix cores (out of place)
+//
+// The real block of a 7-qubit gate (256 x 256 floats, 256 KB) or an 8-qubit
+// one (512 x 512, 1 MB) does not fit in LDS.  It is cut by output rows into S
+// slices of D / S complex rows (the Re and the Im real rows of each), 128 KB
+// of A-operands per slice: S = 2 for 7 targets, 8 for 8.  A persistent
+// workgroup keeps one slice in LDS and walks tiles: it stages the whole
+// 4096-amplitude tile (every column's inputs), multiplies its rows, and
+// writes back only the tile's amplitudes whose top log2(S) canonical target
+// bits equal its slice.  The S workgroups of a tile sit on one XCD and walk
+// in step, so the tile comes from HBM once and from that XCD's L2 S - 1
+// times.  Slices read all of a tile but write disjoint parts of it, so the
+// input must not overlap the output.
+constexpr int kSliceThreads = 512;
+constexpr int kSliceWords = 32768;  // A-operand reals per slice (128 KB)
+constexpr int kSliceGrid = 256;     // one workgroup per CU (160 KB of LDS each)
+
+template <int K, int THREADS> struct SliceShape {
+  static constexpr int W = THREADS / 64;              // wavefronts
+  static constexpr int D = 1 << K;
+  static constexpr int LS = K == 7 ? 1 : 3;          // log2 of the slice count
+  static constexpr int S = 1 << LS;
+  static constexpr int CR = D / S;                    // complex rows per slice
+  static constexpr int RTS = CR / 8;                  // real row tiles per slice (Re, then Im)
+  static constexpr int CT = (kTile >> K) / 16;        // column tiles of 16 groups
+  static constexpr int KS = 2 * D / 4;                // k-steps
+  static constexpr int KSPLIT = (K == 7 ? 1 : 2) * W / 8;  // k-ranges of a row tile
+  static constexpr int WG = W / KSPLIT;               // wavefronts per k-range
+  static constexpr int RPW = RTS * CT / WG;           // row tiles per wavefront
+  static constexpr int NACC = RPW == 1 ? 2 : 1;       // accumulators per row tile
+  static_assert(RTS * KS * 64 == kSliceWords, "slice size");
+  static_assert(RPW >= 1 && (RTS * CT) % WG == 0 && KS % (4 * KSPLIT * NACC) == 0, "split");
+  static_assert(KSPLIT == 1 || (KS / 2) % (KS / KSPLIT) == 0, "a k-range reads xr or xi only");
+};
+
+struct SlicePass {
+  const void* op;       // D x D, row-major, in the caller's qubit order
+  int64_t tiles;        // batch * 2^(n - 12)
+  int64_t in_bstride;   // batch stride of the input (0: one state for every entry)
+  int64_t bstride;      // batch stride of the output: 2^n
+  int log2_tiles;       // n - 12
+  int xcd;              // 256 workgroups, a tile's slices on one XCD
+  int local[kTB];       // ascending state bits of the tile
+  int tpos[8];          // tile positions of the targets, ascending (canonical order)
+  int gpos[8];          // gate-index bit of canonical target m
+  int qpos[8];          // tile positions of the group bits, ascending (12 - K of them)
+};
+
+template <int K, int THREADS>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(1))) void
+dense_slice_kernel(float2* out, const float2* in, SlicePass p) {
+  using Sh = SliceShape<K, THREADS>;
+  using M = Mfma<float>;
+  using V = typename Vec16<float>::type;
+  constexpr int D = Sh::D, RTS = Sh::RTS, CT = Sh::CT, KS = Sh::KS, RPW = Sh::RPW;
+  constexpr int NACC = Sh::NACC, KW = KS / Sh::KSPLIT;  // k-steps per wavefront
+  constexpr int UPT = kTile / 2 / THREADS;          // tile element pairs per thread
+  constexpr int SP = kTile / 2 / Sh::S;                   // element pairs of a slice
+  constexpr int UPS = (SP + THREADS - 1) / THREADS;
+  constexpr int NQ = kTB - K;
+  __shared__ float xs[2 * kTile];  // the tile: Re parts, then Im parts
+  __shared__ __attribute__((aligned(16))) float af[kSliceWords];
+  float* const xr = xs;
+  float* const xi = xs + kTile;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  // ---- this workgroup's slice and tile walk
+  int s, lg, lanes;
+  int64_t base, span;
+  if (p.xcd) {  // workgroup b runs on XCD b % 8
+    const int j = blockIdx.x >> 3;
+    s = j % Sh::S;
+    lg = j / Sh::S;
+    lanes = (kSliceGrid / 8) / Sh::S;
+    span = p.tiles >> 3;
+    base = (blockIdx.x & 7) * span;
+  } else {
+    s = blockIdx.x % Sh::S;
+    lg = blockIdx.x / Sh::S;
+    lanes = gridDim.x / Sh::S;
+    span = p.tiles;
+    base = 0;
+  }
+
+  // ---- A-operands of the slice, layout [lti][ks / 4][lane][ks % 4] (lane l
+  // of k-step ks holds real row 16 * lti + l % 16, real column 4 * ks + l / 16).
+  // Local row tile lti < RTS / 2 holds the Re rows of complex rows CR * s + 16 *
+  // lti + (0..15), the others the Im rows of CR * s + 16 * (lti - RTS / 2) + ....
+  // Each complex entry u of the slice's rows is read once and gives four reals:
+  // Re rows [Re u, -Im u], Im rows [Im u, Re u] (real columns c and D + c).
+  {
+    const float2* U = static_cast<const float2*>(p.op);
+    constexpr int NE = Sh::CR * D;
+    static_assert(NE % (4 * THREADS) == 0, "unrolled staging");
+    auto put = [&](int lti, int r, int col, float val) {
+      const int ks = col >> 2;
+      af[((lti * (KS / 4) + (ks >> 2)) * 64 + r + 16 * (col & 3)) * 4 + (ks & 3)] = val;
+    };
+#pragma unroll 4
+    for (int idx = tid; idx < NE; idx += THREADS) {
+      const int cc = idx & (D - 1), crl = idx >> K;
+      const int cr = Sh::CR * s + crl;
+      int ur = 0, uc = 0;
+      for (int m = 0; m < K; ++m) {
+        ur |= ((cr >> m) & 1) << p.gpos[m];
+        uc |= ((cc >> m) & 1) << p.gpos[m];
+      }
+      const float2 u = U[ur * D + uc];
+      const int lt = crl >> 4, r = crl & 15;
+      put(lt, r, cc, u.x);
+      put(lt, r, D + cc, -u.y);
+      put(lt + RTS / 2, r, cc, u.y);
+      put(lt + RTS / 2, r, D + cc, u.x);
+    }
+  }
+
+  // ---- this wavefront's work: k-range kh, column tile ct, row tiles rg + (RTS / RPW) * j
+  const int kh = wave / Sh::WG;
+  const int ct = (wave % Sh::WG) % CT;
+  const int rg = (wave % Sh::WG) / CT;
+  const int dq = deposit(lane & 15, p.qpos, 4);
+  const int bl = swz(deposit(lane >> 4, p.tpos, 2) | dq);
+  int wl[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) wl[r] = swz(deposit(M::row(lane, r), p.tpos, 4) | dq);
+  const int cb = swz(deposit(16 * ct, p.qpos, NQ));
+  // B-operand positions in xs of this wavefront's k-steps, the same for every
+  // tile (kept in registers): complex column 4 * ks walks the target bits
+  // 2..K-1 of the tile as a subset of tmask (d <- ((d | ~tmask) + 1) & tmask,
+  // wrapping to 0 after the last column); real columns D.. read the Im half
+  int bpos[KW];
+  {
+    int tmask = 0;
+    for (int m = 2; m < K; ++m) tmask |= 1 << p.tpos[m];
+    int d = deposit(4 * ((kh * KW) % (KS / 2)), p.tpos, K);
+    const int b0 = bl ^ cb;
+#pragma unroll
+    for (int t = 0; t < KW; ++t) {
+      const bool im = Sh::KSPLIT == 1 ? t >= KS / 2 : kh * KW >= KS / 2;
+      bpos[t] = opaque((im ? kTile : 0) + (b0 ^ swz(d)));
+      d = ((d | ~tmask) + 1) & tmask;
+    }
+  }
+
+  // ---- tile element pairs: loads cover the tile, stores the slice's part
+  int64_t dep_t = 0;
+  for (int i = 1; i < kTB; ++i) dep_t |= int64_t(((2 * tid) >> i) & 1) << p.local[i];
+  auto dep_e = [&](int e) {
+    int64_t d = 0;
+    for (int i = 1; i < kTB; ++i) d |= int64_t((e >> i) & 1) << p.local[i];
+    return d;
+  };
+  // slice bits: the top LS canonical targets (tile positions >= 5, so never
+  // bit 0: an element pair lies in one slice)
+  const int* spos = p.tpos + (K - Sh::LS);
+  int sbits = 0;
+  for (int j = 0; j < Sh::LS; ++j) sbits |= ((s >> j) & 1) << spos[j];
+  int64_t st_off[UPS];
+  int st_lds[UPS];
+#pragma unroll
+  for (int u = 0; u < UPS; ++u) {
+    const int pi = tid + THREADS * u;
+    const int e = (pi < SP ? int(insert_zero_bits(uint64_t(2 * pi), spos, Sh::LS)) : 0) | sbits;
+    st_off[u] = dep_e(e);
+    st_lds[u] = swz(e);
+  }
+  const int swz_t = swz(2 * tid);
+  auto tile_at = [&](int64_t it) -> int64_t {  // -1 past the end
+    const int64_t t = lg + it * lanes;
+    return t < span ? base + t : -1;
+  };
+  auto origin_of = [&](int64_t t, int64_t bstride) -> int64_t {
+    const int64_t b = t >> p.log2_tiles;
+    const uint64_t tin = uint64_t(t) & ((uint64_t(1) << p.log2_tiles) - 1);
+    return b * bstride + int64_t(insert_zero_bits(tin, p.local, kTB));
+  };
+
+  Pair<float2> v[UPT];
+  int64_t tile = tile_at(0);
+  if (tile >= 0) {
+    const int64_t o = origin_of(tile, p.in_bstride) + dep_t;
+#pragma unroll
+    for (int u = 0; u < UPT; ++u)
+      v[u] = *reinterpret_cast<const Pair<float2>*>(in + o + dep_e(2 * THREADS * u));
+  }
+  for (int64_t it = 0; tile >= 0; ++it) {
+    const int st = opaque(swz_t);
+#pragma unroll
+    for (int u = 0; u < UPT; ++u) {
+      const int e = st ^ swz(2 * THREADS * u);
+      xr[e] = v[u].v[0].x;
+      xi[e] = v[u].v[0].y;
+      xr[e ^ 1] = v[u].v[1].x;
+      xi[e ^ 1] = v[u].v[1].y;
+    }
+    __syncthreads();  // also: the A-operands are staged (first tile)
+    const int64_t next = tile_at(it + 1);
+    if (next >= 0) {
+      const int64_t o = origin_of(next, p.in_bstride) + dep_t;
+#pragma unroll
+      for (int u = 0; u < UPT; ++u)
+        v[u] = *reinterpret_cast<const Pair<float2>*>(in + o + dep_e(2 * THREADS * u));
+    }
+    // ---- products: acc[j][a] over this wavefront's k-steps (a: even / odd k-step chains)
+    typename M::acc_t acc[RPW][NACC];
+#pragma unroll
+    for (int j = 0; j < RPW; ++j)
+#pragma unroll
+      for (int a = 0; a < NACC; ++a) acc[j][a] = typename M::acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < KW / 4; ++kk) {
+      V a[RPW];
+#pragma unroll
+      for (int j = 0; j < RPW; ++j) {
+        const int lti = rg + (RTS / RPW) * j;
+        a[j] = *reinterpret_cast<const V*>(
+            af + ((lti * (KS / 4) + kh * (KW / 4) + kk) * 64 + lane) * 4);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float b = xs[bpos[4 * kk + q]];
+#pragma unroll
+        for (int j = 0; j < RPW; ++j)
+          acc[j][q % NACC] = M::mma(a[j][q], b, acc[j][q % NACC]);
+      }
+    }
+    __syncthreads();  // every B-operand is read: results may replace the inputs
+    // ---- results into the tile (k-range 1 first, then range 0 adds)
+#pragma unroll
+    for (int round = Sh::KSPLIT - 1; round >= 0; --round) {
+      if (kh == round) {
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+          const int lti = rg + (RTS / RPW) * j;
+          float* dst = lti < RTS / 2 ? xr : xi;
+          const int sj = swz(deposit(Sh::CR * s + 16 * (lti % (RTS / 2)), p.tpos, K)) ^ cb;
+          typename M::acc_t r4 = acc[j][0];
+          if constexpr (NACC == 2) r4 += acc[j][1];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int e = opaque(wl[r]) ^ sj;
+            dst[e] = round == Sh::KSPLIT - 1 ? r4[r] : dst[e] + r4[r];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // ---- the slice's amplitudes back to the state
+    const int64_t o = origin_of(tile, p.bstride);
+#pragma unroll
+    for (int u = 0; u < UPS; ++u) {
+      if (tid + THREADS * u < SP) {
+        const int e = opaque(st_lds[u]);
+        Pair<float2> w;
+        w.v[0].x = xr[e];
+        w.v[0].y = xi[e];
+        w.v[1].x = xr[e ^ 1];
+        w.v[1].y = xi[e ^ 1];
+        *reinterpret_cast<Pair<float2>*>(out + o + st_off[u]) = w;
+      }
+    }
+    __syncthreads();
+    tile = next;
+  }
+}
+
 // Launch geometry: complex64 256 threads and 3 gates (80 KB LDS, two
 // workgroups per CU); complex128 512 threads and 2 gates (128 KB, one).
 constexpr int kThreads64 = 256, kThreads128 = 512;
@@ -459,6 +727,78 @@ int apply_dense_six(void* out, const void* in, const void* op, int n, const int*
   for (int m = 0; m < 6; ++m) g.qubits[m] = qubits[m];
   g.data = op;
   return launch_pass(out, in, in_bstride, UA_COMPLEX64, n, batch, &g, 1, s);
+}
+
+// A single 7- or 8-qubit complex64 gate of apply_operator on the matrix cores
+// (one operator for every batch entry), out of place.  True when it applies:
+// complex64, at least kMinBlockQubits qubits, pair-aligned buffers and
+// strides, and an input that does not overlap the output (batch >= 1).
+bool slice_applies(int dtype, int n, int k, const void* out, const void* in, int64_t in_bstride,
+                   int64_t batch) {
+  if (!((k == 7 || k == 8) && dtype == UA_COMPLEX64 && n >= kMinBlockQubits && batch >= 1 &&
+        pair_aligned(out, dtype) && pair_aligned(in, dtype) && in_bstride % 2 == 0))
+    return false;
+  const uintptr_t o0 = reinterpret_cast<uintptr_t>(out);
+  const uintptr_t o1 = o0 + uintptr_t(batch << n) * sizeof(float2);
+  const uintptr_t i0 = reinterpret_cast<uintptr_t>(in);
+  const uintptr_t i1 = i0 + uintptr_t((batch - 1) * in_bstride + (int64_t(1) << n)) * sizeof(float2);
+  return o1 <= i0 || i1 <= o0;
+}
+
+int apply_dense_slices(void* out, const void* in, const void* op, int n, int k,
+                       const int* qubits, int64_t batch, int64_t in_bstride, hipStream_t s) {
+  UA_CHECK_ARG(slice_applies(UA_COMPLEX64, n, k, out, in, in_bstride, batch),
+               "internal: sliced matrix-core path for %d targets on %d qubits", k, n);
+  SlicePass p{};
+  p.op = op;
+  p.log2_tiles = n - kTB;
+  p.tiles = batch << p.log2_tiles;
+  p.in_bstride = in_bstride;
+  p.bstride = int64_t(1) << n;
+  int tb[8], gp[8];
+  for (int m = 0; m < k; ++m) {
+    tb[m] = n - 1 - qubits[m];
+    gp[m] = k - 1 - m;
+  }
+  for (int i = 1; i < k; ++i)  // canonical order: ascending state bit
+    for (int j = i; j > 0 && tb[j - 1] > tb[j]; --j) {
+      int t0 = tb[j]; tb[j] = tb[j - 1]; tb[j - 1] = t0;
+      int g0 = gp[j]; gp[j] = gp[j - 1]; gp[j - 1] = g0;
+    }
+  // tile: the targets, state bits 0 and 1, then the lowest other bits
+  uint64_t uni = 3, tmask_state = 0;
+  for (int m = 0; m < k; ++m) tmask_state |= uint64_t(1) << tb[m];
+  uni |= tmask_state;
+  for (int bit = 0; __builtin_popcountll(uni) < kTB; ++bit) uni |= uint64_t(1) << bit;
+  for (int i = 0; i < kTB; ++i) {
+    p.local[i] = __builtin_ctzll(uni);
+    uni &= uni - 1;
+  }
+  int nq = 0, nt = 0;
+  for (int i = 0; i < kTB; ++i) {
+    if ((tmask_state >> p.local[i]) & 1) {
+      p.tpos[nt] = i;
+      p.gpos[nt] = gp[nt];
+      UA_CHECK_ARG(tb[nt] == p.local[i], "internal: target order");
+      ++nt;
+    } else {
+      p.qpos[nq++] = i;
+    }
+  }
+  UA_CHECK_ARG(nt == k && nq == kTB - k, "internal: tile of %d targets", nt);
+  const int S = k == 7 ? SliceShape<7, kSliceThreads>::S : SliceShape<8, kSliceThreads>::S;
+  // 256 workgroups with each tile's S slices on one XCD, or (few tiles, or a
+  // tile count that is not a multiple of 8) up to 256 dealt in slice order
+  p.xcd = p.tiles % 8 == 0 && p.tiles * S >= kSliceGrid;
+  const int64_t grid = p.xcd ? kSliceGrid : (p.tiles * S < kSliceGrid ? p.tiles * S : kSliceGrid);
+  UA_CHECK_ARG(grid >= S && grid % S == 0, "internal: grid %lld", (long long)grid);
+  float2* o = static_cast<float2*>(out);
+  const float2* i = static_cast<const float2*>(in);
+  if (k == 7)
+    dense_slice_kernel<7, kSliceThreads><<<unsigned(grid), kSliceThreads, 0, s>>>(o, i, p);
+  else
+    dense_slice_kernel<8, kSliceThreads><<<unsigned(grid), kSliceThreads, 0, s>>>(o, i, p);
+  return check_launch("dense_slice_kernel");
 }
 
 }  // namespace ua

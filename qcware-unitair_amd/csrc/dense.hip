@@ -159,6 +159,11 @@ bool aligned(const void* ptr, int64_t stride_elems, size_t esize, size_t align) 
 bool six_applies(int dtype, int n, int k, const void* out, const void* in, int64_t in_bstride);
 int apply_dense_six(void* out, const void* in, const void* op, int n, const int* qubits,
                     int64_t batch, int64_t in_bstride, hipStream_t s);
+// Single 7/8-qubit complex64 gates on the matrix cores, out of place (blocks.hip).
+bool slice_applies(int dtype, int n, int k, const void* out, const void* in, int64_t in_bstride,
+                   int64_t batch);
+int apply_dense_slices(void* out, const void* in, const void* op, int n, int k,
+                       const int* qubits, int64_t batch, int64_t in_bstride, hipStream_t s);
 
 // Tiled dense gates on 6..kMaxWideGateK qubits (wide.hip).
 constexpr int kMaxWideGateK = 10;
@@ -262,6 +267,9 @@ int apply_operator_impl(void* out, const void* state, const void* op, int dtype,
 
   if (g_use_mfma && op_bstride == 0 && six_applies(dtype, n, k, out, state, state_bstride))
     return apply_dense_six(out, state, op, n, qubits, batch, state_bstride, s);
+  if (g_use_mfma && op_bstride == 0 &&
+      slice_applies(dtype, n, k, out, state, state_bstride, batch))
+    return apply_dense_slices(out, state, op, n, k, qubits, batch, state_bstride, s);
   if (wide_applies(n, k)) {  // 6..10 targets: tiled, in place or not
     int tbc[kMaxWideGateK];
     for (int m = 0; m < k; ++m) tbc[m] = n - 1 - qubits[m];

@@ -368,6 +368,49 @@ def test_six_qubit_complex64_gates_on_matrix_cores(device, n, sb):
     assert torch.equal(x.reshape(out.shape), out)
 
 
+@pytest.mark.parametrize("k,n,sb", [(7, 14, ()), (7, 14, (3,)), (8, 14, (2,)), (8, 15, (3,)),
+                                    (8, 17, ()), (7, 19, ())])
+def test_seven_eight_qubit_complex64_gates_on_matrix_cores(device, k, n, sb):
+    """One 7- or 8-qubit operator on complex64 states of at least 14 qubits,
+    out of place, runs on the matrix cores with the real block cut into row
+    slices (2 or 8 workgroups per tile; 17 and 19 qubits take the XCD-grouped
+    walk, 14 and 15 qubits and batches of 3 the plain one): against the
+    oracle, against the tiled vector kernel (matrix cores switched off), with
+    one state broadcast over a batch (input batch stride 0), and in place
+    (which stays on the vector kernel)."""
+    from unitair_amd import _lib
+    dt = torch.complex64
+    rng = np.random.default_rng(100 * k + n + len(sb))
+    qubits = [int(q) for q in rng.permutation(n)[:k]]
+    op = rand((2 ** k, 2 ** k), dt, rng) / np.sqrt(2 ** k)
+    st = rand(sb + (2 ** n,), dt, rng)
+    opd, std = to_dev(op, dt, device), to_dev(st, dt, device)
+    out = sim.apply_operator(opd, qubits, std)
+    ref = U.apply_operator(cast_np(op, dt), qubits, cast_np(st, dt))
+    assert rel_err(out.cpu().numpy(), ref) < TOL[dt]
+    lib = _lib.load()
+    lib.ua_set_tuning(_lib.UA_TUNE_MFMA, 0)
+    try:
+        vec = sim.apply_operator(opd, qubits, std)
+    finally:
+        lib.ua_set_tuning(_lib.UA_TUNE_MFMA, 1)
+    assert rel_err(out.cpu().numpy(), vec.cpu().numpy().astype(np.complex128)) < TOL[dt]
+    # one state for three batch entries through the C ABI
+    one = std.reshape(-1, 2 ** n)[:1].contiguous()
+    bc = torch.empty((3, 2 ** n), dtype=dt, device=device)
+    rc = lib.ua_apply_operator(_lib.ptr(bc), _lib.ptr(one), _lib.ptr(opd), _lib.UA_COMPLEX64, n,
+                               k, _lib.int_array(qubits), 3, 0, 0, _lib.stream_handle(device))
+    _lib.check(rc, "broadcast state")
+    for b in range(3):
+        assert torch.equal(bc[b], out.reshape(-1, 2 ** n)[0])
+    x = std.clone().reshape(-1, 2 ** n)
+    rc = lib.ua_apply_operator(_lib.ptr(x), _lib.ptr(x), _lib.ptr(opd), _lib.UA_COMPLEX64, n, k,
+                               _lib.int_array(qubits), x.shape[0], 2 ** n, 0,
+                               _lib.stream_handle(device))
+    _lib.check(rc, "in place")
+    assert torch.equal(x.reshape(out.shape), vec)
+
+
 @pytest.mark.parametrize("dt,n,k", WIDE)
 def test_wide_gates_match_oracle_in_and_out_of_place(device, dt, n, k):
     """6..10-qubit operators on states of at least 12 qubits take the tiled
