@@ -84,39 +84,71 @@ template <typename R> struct Vec16;  // 16 bytes of A-operands
 template <> struct Vec16<float> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); };
 
-// Multiplies every group of the tile by gate gi (K targets).  B-operands come
-// from xr / xi, A-operands from af; results overwrite the wavefront's own
-// groups.
-template <typename T, int THREADS, int K>
-__device__ __forceinline__ void apply_gate(typename RealOf<T>::type* xr,
-                                           typename RealOf<T>::type* xi,
-                                           const typename RealOf<T>::type* af,
-                                           const BlockGate& g, int lane, int wave, int bl,
-                                           const int (&wl)[4]) {
+// How apply_gate splits a K-target gate over the workgroup's wavefronts.
+template <typename T, int THREADS, int K> struct GateShape {
   using R = typename RealOf<T>::type;
-  using M = Mfma<R>;
-  using V = typename Vec16<R>::type;
-  constexpr int W = THREADS / 64;
-  constexpr int D = 1 << K;
-  constexpr int RT = 2 * D / 16;           // row tiles of the real block
-  constexpr int KS = 2 * D / 4;            // k-steps
-  constexpr int AV = 16 / sizeof(R);       // A values per 16-byte read
-  constexpr int CT = (kTile >> K) / 16;    // column tiles (16 groups each)
+  static constexpr int W = THREADS / 64;
+  static constexpr int D = 1 << K;
+  static constexpr int RT = 2 * D / 16;           // row tiles of the real block
+  static constexpr int KS = 2 * D / 4;            // k-steps
+  static constexpr int AV = 16 / sizeof(R);       // A values per 16-byte read
+  static constexpr int CT = (kTile >> K) / 16;    // column tiles (16 groups each)
   // With fewer column tiles than wavefronts, WPC wavefronts share a column
   // tile, each taking RTW of its row tiles (then the results are written
   // only after every wavefront has read its B-operands).
-  constexpr int WPC = W > CT ? W / CT : 1;
-  constexpr int CPW = W > CT ? 1 : CT / W;  // column tiles per wavefront
-  constexpr int RTW = RT / WPC;             // row tiles per wavefront
+  static constexpr int WPC = W > CT ? W / CT : 1;
+  static constexpr int CPW = W > CT ? 1 : CT / W;  // column tiles per wavefront
+  static constexpr int RTW = RT / WPC;             // row tiles per wavefront
+  static constexpr int NPOS = KS * CPW;            // B-operand reads per wavefront
   static_assert(CT % (W / WPC) == 0 && RT % WPC == 0, "wavefront split");
   static_assert(WPC == 1 || RTW <= RT / 2, "a wavefront's rows stay in one half");
-  const int nq = kTB - K;
-  const int cw = wave % (W / WPC);          // column-tile lane of this wavefront
+};
+
+// Swizzled offset of column tile c of this wavefront for gate g.
+template <typename T, int THREADS, int K>
+__device__ __forceinline__ int column_offset(const BlockGate& g, int wave, int c) {
+  using Sh = GateShape<T, THREADS, K>;
+  const int cw = wave % (Sh::W / Sh::WPC);
+  return swz(deposit(16 * (cw + (Sh::W / Sh::WPC) * c), g.qpos, kTB - K));
+}
+
+// B-operand positions of gate g read by this wavefront, in apply_gate's
+// order (k-step ks, column tile c at ks * CPW + c), as offsets from the Re
+// half of the tile (the Im half starts at kTile).  They are the same for
+// every tile, so the kernel forms them once and keeps them in registers:
+// recomputed per tile, each read waited on its own address chain.
+template <typename T, int THREADS, int K>
+__device__ __forceinline__ void gate_positions(const BlockGate& g, int lane, int wave, int* pos) {
+  using Sh = GateShape<T, THREADS, K>;
+  const int bl = swz(deposit(lane >> 4, g.tpos, 2) | deposit(lane & 15, g.qpos, 4));
+#pragma unroll
+  for (int ks = 0; ks < Sh::KS; ++ks) {
+    const int sj = swz(deposit(4 * (ks % (Sh::KS / 2)), g.tpos, K));
+#pragma unroll
+    for (int c = 0; c < Sh::CPW; ++c)
+      pos[ks * Sh::CPW + c] = opaque((ks >= Sh::KS / 2 ? kTile : 0) +
+                                     (bl ^ sj ^ column_offset<T, THREADS, K>(g, wave, c)));
+  }
+}
+
+// Multiplies every group of the tile by gate gi (K targets).  B-operands come
+// from the tile x (Re half, then Im half) at the positions pos, A-operands
+// from af; results overwrite the wavefront's own groups.
+template <typename T, int THREADS, int K>
+__device__ __forceinline__ void apply_gate(typename RealOf<T>::type* x,
+                                           const typename RealOf<T>::type* af,
+                                           const BlockGate& g, int lane, int wave,
+                                           const int* pos, const int (&wl)[4]) {
+  using Sh = GateShape<T, THREADS, K>;
+  using R = typename RealOf<T>::type;
+  using M = Mfma<R>;
+  using V = typename Vec16<R>::type;
+  constexpr int RT = Sh::RT, KS = Sh::KS, AV = Sh::AV, W = Sh::W, WPC = Sh::WPC;
+  constexpr int CPW = Sh::CPW, RTW = Sh::RTW;
   const int t0 = (wave / (W / WPC)) * RTW;  // first row tile
-  bl = opaque(bl);
   int cb[CPW];  // swizzled offsets of this wavefront's column tiles
 #pragma unroll
-  for (int c = 0; c < CPW; ++c) cb[c] = swz(deposit(16 * (cw + (W / WPC) * c), g.qpos, nq));
+  for (int c = 0; c < CPW; ++c) cb[c] = column_offset<T, THREADS, K>(g, wave, c);
   typename M::acc_t acc[CPW][RTW];
 #pragma unroll
   for (int c = 0; c < CPW; ++c)
@@ -131,11 +163,9 @@ __device__ __forceinline__ void apply_gate(typename RealOf<T>::type* xr,
 #pragma unroll
     for (int q = 0; q < AV; ++q) {
       const int ks = kk * AV + q;
-      const R* src = (ks >= KS / 2) ? xi : xr;
-      const int sj = swz(deposit(4 * (ks % (KS / 2)), g.tpos, K));
       R b[CPW];
 #pragma unroll
-      for (int c = 0; c < CPW; ++c) b[c] = src[bl ^ sj ^ cb[c]];
+      for (int c = 0; c < CPW; ++c) b[c] = x[pos[ks * CPW + c]];
 #pragma unroll
       for (int ti = 0; ti < RTW; ++ti)
 #pragma unroll
@@ -149,7 +179,7 @@ __device__ __forceinline__ void apply_gate(typename RealOf<T>::type* xr,
 #pragma unroll
   for (int tl = 0; tl < RTW; ++tl) {
     const int ti = t0 + tl;
-    R* dst = (ti >= RT / 2) ? xi : xr;
+    R* dst = (ti >= RT / 2) ? x + kTile : x;
     const int sj = swz(deposit(16 * (ti % (RT / 2)), g.tpos, K));
 #pragma unroll
     for (int c = 0; c < CPW; ++c)
@@ -167,8 +197,10 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FRAG ==
   using R = typename RealOf<T>::type;
   constexpr int UPT = kTile / 2 / THREADS;  // element pairs per thread
   constexpr int AV = 16 / sizeof(R);
-  __shared__ R xr[kTile], xi[kTile];
+  __shared__ R xs[2 * kTile];  // the tile: Re parts, then Im parts
   __shared__ __attribute__((aligned(16))) R af[MAXG][FRAG];
+  R* const xr = xs;
+  R* const xi = xs + kTile;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   // ---- A-operands of every gate, layout [ti][ks / AV][lane][ks % AV]
@@ -192,13 +224,21 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FRAG ==
       af[gi][idx] = real_block(U[ur * D + uc], row >= D, col >= D);
     }
   }
-  // ---- per-gate lane offsets: B reads (bl) and result writes (wl)
-  int bl[MAXG], wl[MAXG][4];
+  // ---- per-gate B-operand positions and result-write lane offsets (wl)
+  constexpr int NPOS = FRAG == kFragWords6 ? GateShape<T, THREADS, 6>::NPOS
+                                           : GateShape<T, THREADS, 5>::NPOS;
+  static_assert(FRAG == kFragWords6 || GateShape<T, THREADS, 4>::NPOS == NPOS, "positions");
+  int pos[MAXG][NPOS], wl[MAXG][4];
 #pragma unroll
   for (int gi = 0; gi < MAXG; ++gi) {
     const BlockGate& g = p.g[gi];
+    if constexpr (FRAG == kFragWords6)
+      gate_positions<T, THREADS, 6>(g, lane, wave, pos[gi]);
+    else if (g.k == 5)
+      gate_positions<T, THREADS, 5>(g, lane, wave, pos[gi]);
+    else
+      gate_positions<T, THREADS, 4>(g, lane, wave, pos[gi]);
     const int dq = deposit(lane & 15, g.qpos, 4);
-    bl[gi] = swz(deposit(lane >> 4, g.tpos, 2) | dq);
 #pragma unroll
     for (int r = 0; r < 4; ++r) wl[gi][r] = swz(deposit(Mfma<R>::row(lane, r), g.tpos, 4) | dq);
   }
@@ -258,11 +298,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FRAG ==
     for (int gi = 0; gi < MAXG; ++gi) {
       if (gi < p.num_gates) {
         if constexpr (FRAG == kFragWords6)
-          apply_gate<T, THREADS, 6>(xr, xi, af[gi], p.g[gi], lane, wave, bl[gi], wl[gi]);
+          apply_gate<T, THREADS, 6>(xs, af[gi], p.g[gi], lane, wave, pos[gi], wl[gi]);
         else if (p.g[gi].k == 5)
-          apply_gate<T, THREADS, 5>(xr, xi, af[gi], p.g[gi], lane, wave, bl[gi], wl[gi]);
+          apply_gate<T, THREADS, 5>(xs, af[gi], p.g[gi], lane, wave, pos[gi], wl[gi]);
         else
-          apply_gate<T, THREADS, 4>(xr, xi, af[gi], p.g[gi], lane, wave, bl[gi], wl[gi]);
+          apply_gate<T, THREADS, 4>(xs, af[gi], p.g[gi], lane, wave, pos[gi], wl[gi]);
         __syncthreads();
       }
     }
