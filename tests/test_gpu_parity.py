@@ -369,12 +369,12 @@ def test_six_qubit_complex64_gates_on_matrix_cores(device, n, sb):
 
 
 @pytest.mark.parametrize("k,n,sb", [(7, 14, ()), (7, 14, (3,)), (8, 14, (2,)), (8, 15, (3,)),
-                                    (8, 17, ()), (7, 19, ())])
+                                    (8, 17, ()), (7, 19, ()), (8, 14, (8,)), (7, 15, (16,))])
 def test_seven_eight_qubit_complex64_gates_on_matrix_cores(device, k, n, sb):
     """One 7- or 8-qubit operator on complex64 states of at least 14 qubits,
     out of place, runs on the matrix cores with the real block cut into row
-    slices (2 or 8 workgroups per tile; 17 and 19 qubits take the XCD-grouped
-    walk, 14 and 15 qubits and batches of 3 the plain one): against the
+    slices (2 or 8 workgroups per tile; 17 and 19 qubits and the batches of 8
+    and 16 take the XCD-grouped walk, the other cases the plain one): against the
     oracle, against the tiled vector kernel (matrix cores switched off), with
     one state broadcast over a batch (input batch stride 0), and in place
     (which stays on the vector kernel)."""
