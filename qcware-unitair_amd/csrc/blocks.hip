@@ -550,7 +550,7 @@ dense_slice_kernel(float2* out, const float2* in, SlicePass p) {
       }
     }
     __syncthreads();  // every B-operand is read: results may replace the inputs
-    // ---- results into the tile (k-range 1 first, then range 0 adds)
+    // ---- results into the tile: the last k-range stores, the others add in turn
 #pragma unroll
     for (int round = Sh::KSPLIT - 1; round >= 0; --round) {
       if (kh == round) {
